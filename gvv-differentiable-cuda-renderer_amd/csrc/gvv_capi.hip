@@ -1,0 +1,318 @@
+// gvv_capi.hip -- C ABI of the MI355X rasterizer (declared in include/gvv_renderer.h).
+//
+// One handle corresponds to one reference op instance: the topology and attributes of
+// REGISTER_OP("CudaRendererGpu") / ("CudaRendererGradGpu")
+// (reference cpp/src/TensorflowOperators/CudaRenderer/CudaRenderer.cpp:5-33,
+//  CudaRendererGrad.cpp:6-39).  The handle owns device copies of the faces, UVs and the
+// vertex->face CSR, the texel table of the normal map, and scratch buffers that grow with
+// the batch size.  Tensor arguments are borrowed device pointers; every launch goes to the
+// caller's stream.  Errors return non-zero and leave a message for gvv_last_error().
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "gvv_kernels.hpp"
+#include "../../include/gvv_renderer.h"
+
+namespace {
+
+using namespace gvv;
+
+thread_local std::string g_err;
+int fail(const std::string& m) { g_err = m; return 1; }
+
+#define GVV_CHECK(expr)                                                              \
+  do {                                                                               \
+    hipError_t _e = (expr);                                                          \
+    if (_e != hipSuccess) return fail(std::string(#expr) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  hipError_t reserve(size_t count) {
+    if (count <= n) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr; n = 0;
+    hipError_t e = hipMalloc((void**)&p, sizeof(T) * (count ? count : 1));
+    if (e == hipSuccess) n = count;
+    return e;
+  }
+  void release() { if (p) (void)hipFree(p); p = nullptr; n = 0; }
+};
+
+struct Handle {
+  int N = 0, F = 0, C = 0, W = 0, H = 0;
+  int albedo = 0, shading = 0, image_fs = 1, texture_fs = 1, normal_map = 0;
+  int device = 0;
+  std::vector<int32_t> faces;
+  std::vector<float> uv;
+  DevBuf<int32_t> d_faces, d_csr_start, d_csr_count, d_csr_list;
+  DevBuf<float> d_uv;
+  int uvW = -1, uvH = -1;
+  DevBuf<UvTexel> d_uv_table;
+  // scratch
+  DevBuf<Camera> cams;
+  DevBuf<float> fnormal;
+  DevBuf<float4> proj;
+  DevBuf<unsigned long long> keys;
+  DevBuf<float> g_vn;
+  DevBuf<float> sh_partial;
+  Profiler prof;
+  std::mutex mu;
+
+  ~Handle() {
+    d_faces.release(); d_csr_start.release(); d_csr_count.release(); d_csr_list.release();
+    d_uv.release(); d_uv_table.release(); cams.release(); fnormal.release(); proj.release();
+    keys.release(); g_vn.release(); sh_partial.release();
+  }
+};
+
+// Vertex->face lists in increasing face order, each face once per vertex
+// (reference CUDABasedRasterization.cpp:125-154), built by counting sort in O(N + F).
+void build_csr(int N, const std::vector<int32_t>& faces, std::vector<int32_t>& start,
+               std::vector<int32_t>& count, std::vector<int32_t>& list) {
+  const int F = (int)(faces.size() / 3);
+  start.assign(N, 0); count.assign(N, 0);
+  for (int f = 0; f < F; ++f) {
+    int a = faces[3 * f], b = faces[3 * f + 1], c = faces[3 * f + 2];
+    count[a]++;
+    if (b != a) count[b]++;
+    if (c != a && c != b) count[c]++;
+  }
+  int run = 0;
+  for (int v = 0; v < N; ++v) { start[v] = run; run += count[v]; }
+  list.assign(run, 0);
+  std::vector<int32_t> fill(N, 0);
+  for (int f = 0; f < F; ++f) {
+    int a = faces[3 * f], b = faces[3 * f + 1], c = faces[3 * f + 2];
+    list[start[a] + fill[a]++] = f;
+    if (b != a) list[start[b] + fill[b]++] = f;
+    if (c != a && c != b) list[start[c] + fill[c]++] = f;
+  }
+}
+
+// Texel -> (face, barycentrics) table (reference CUDABasedRasterization.cpp:237-298): each
+// face's UV triangle is intersected in texel space by the ray from (x+.5, y+.5, 1) along -z;
+// later faces overwrite earlier ones; uncovered texels keep (face 0, 0, 0, 0).
+void build_uv_table(const Handle& h, int texW, int texH, std::vector<UvTexel>& table) {
+  table.assign((size_t)texW * texH, UvTexel{0, 0.f, 0.f, 0.f});
+  const float* uv = h.uv.data();
+  const V3 dir = v3(0.f, 0.f, -1.f);
+  for (int f = 0; f < h.F; ++f) {
+    V3 t0 = v3((float)texW * uv[f * 6 + 0], (float)texH * (1.f - uv[f * 6 + 1]), 0.f);
+    V3 t1 = v3((float)texW * uv[f * 6 + 2], (float)texH * (1.f - uv[f * 6 + 3]), 0.f);
+    V3 t2 = v3((float)texW * uv[f * 6 + 4], (float)texH * (1.f - uv[f * 6 + 5]), 0.f);
+    int xMin = sat_i32(fmaxf(fminf(t0.x, fminf(t1.x, t2.x)) - 2.f, 0.f));
+    int xMax = sat_i32(fminf(fmaxf(t0.x, fmaxf(t1.x, t2.x)) + 2.f, (float)texW));
+    int yMin = sat_i32(fmaxf(fminf(t0.y, fminf(t1.y, t2.y)) - 2.f, 0.f));
+    int yMax = sat_i32(fminf(fmaxf(t0.y, fmaxf(t1.y, t2.y)) + 2.f, (float)texH));
+    TriSetup ts = tri_setup(t0, t1, t2);
+    for (int x = xMin; x < xMax; ++x)
+      for (int y = yMin; y < yMax; ++y) {
+        V3 o = v3(x + 0.5f, y + 0.5f, 1.f);
+        V3 abc = ray_barycentric(ts, o / 1000.f, dir);
+        if (abc.x != -1.f && abc.y != -1.f && abc.z != -1.f)
+          table[(size_t)y * texW + x] = UvTexel{f, abc.x, abc.y, abc.z};
+      }
+  }
+}
+
+int check_device(const Handle* h) {
+  int dev = -1;
+  GVV_CHECK(hipGetDevice(&dev));
+  if (dev != h->device) return fail("handle was created on device " + std::to_string(h->device) +
+                                    " but the current device is " + std::to_string(dev));
+  return 0;
+}
+
+inline int tiles(int n) { return (n + 15) / 16; }
+
+}  // namespace
+
+extern "C" {
+
+const char* gvv_last_error(void) { return g_err.c_str(); }
+
+int gvv_version(void) { return GVV_RENDERER_ABI_VERSION; }
+
+int gvv_create(const int32_t* faces, int64_t n_face_idx, const float* texcoords, int64_t n_texcoords,
+               int32_t n_vertices, int32_t n_cameras, int32_t width, int32_t height, int32_t albedo_mode,
+               int32_t shading_mode, int32_t image_filter_size, int32_t texture_filter_size,
+               int32_t compute_normal_map, void** handle) {
+  if (!handle) return fail("handle pointer is null");
+  *handle = nullptr;
+  if (!faces || n_face_idx <= 0 || n_face_idx % 3 != 0) return fail("faces must hold 3*F > 0 vertex indices");
+  const int64_t F = n_face_idx / 3;
+  if (F > (1ll << 30)) return fail("too many faces");
+  if (!texcoords || n_texcoords != 6 * F) return fail("texture_coordinates must hold 6*F values (u,v per face corner)");
+  if (n_vertices <= 0) return fail("number_of_vertices must be > 0");
+  if (n_cameras <= 0) return fail("number_of_cameras must be > 0");
+  if (width <= 0 || height <= 0) return fail("render_resolution_u/v must be > 0");
+  if (albedo_mode < 0 || albedo_mode > 4) return fail("invalid albedo mode");
+  if (shading_mode < 0 || shading_mode > 1) return fail("invalid shading mode");
+  if (image_filter_size <= 0) return fail("image_filter_size must be > 0");
+  if (texture_filter_size <= 0) return fail("texture_filter_size must be > 0");
+  for (int64_t i = 0; i < n_face_idx; ++i)
+    if (faces[i] < 0 || faces[i] >= n_vertices)
+      return fail("face index " + std::to_string(faces[i]) + " out of range [0, " + std::to_string(n_vertices) + ")");
+  Handle* h = new Handle();
+  h->N = n_vertices; h->F = (int)F; h->C = n_cameras; h->W = width; h->H = height;
+  h->albedo = albedo_mode;
+  // foregroundMask forces shadeless (CudaRenderer.cpp:72-76, CudaRendererGrad.cpp:78-82)
+  h->shading = albedo_mode == kForegroundMask ? kShadeless : shading_mode;
+  h->image_fs = image_filter_size; h->texture_fs = texture_filter_size;
+  h->normal_map = compute_normal_map ? 1 : 0;
+  h->faces.assign(faces, faces + n_face_idx);
+  h->uv.assign(texcoords, texcoords + n_texcoords);
+  std::vector<int32_t> start, count, list;
+  build_csr(h->N, h->faces, start, count, list);
+  hipError_t e = hipGetDevice(&h->device);
+  if (e == hipSuccess) e = h->d_faces.reserve(h->faces.size());
+  if (e == hipSuccess) e = h->d_uv.reserve(h->uv.size());
+  if (e == hipSuccess) e = h->d_csr_start.reserve(start.size());
+  if (e == hipSuccess) e = h->d_csr_count.reserve(count.size());
+  if (e == hipSuccess) e = h->d_csr_list.reserve(list.size());
+  if (e == hipSuccess) e = hipMemcpy(h->d_faces.p, h->faces.data(), sizeof(int32_t) * h->faces.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(h->d_uv.p, h->uv.data(), sizeof(float) * h->uv.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(h->d_csr_start.p, start.data(), sizeof(int32_t) * start.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(h->d_csr_count.p, count.data(), sizeof(int32_t) * count.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess && !list.empty())
+    e = hipMemcpy(h->d_csr_list.p, list.data(), sizeof(int32_t) * list.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    delete h;
+    return fail(std::string("gvv_create: ") + hipGetErrorString(e));
+  }
+  *handle = h;
+  return 0;
+}
+
+void gvv_destroy(void* handle) { delete (Handle*)handle; }
+
+int gvv_forward(void* handle, int32_t B, int32_t texH, int32_t texW, const float* vpos, const float* vcol,
+                const float* tex, const float* sh, const float* extr, const float* intr, float* bary,
+                int32_t* face, float* render, float* vnormal, float* normal_map, void* hip_stream) {
+  Handle* h = (Handle*)handle;
+  if (!h) return fail("null handle");
+  std::lock_guard<std::mutex> lock(h->mu);
+  if (check_device(h)) return 1;
+  if (B <= 0) return fail("batch size must be > 0");
+  if (texH <= 0 || texW <= 0) return fail("texture height/width must be > 0");
+  if (!vpos || !vcol || !tex || !sh || !extr || !intr || !bary || !face || !render || !vnormal)
+    return fail("null tensor pointer");
+  if (h->normal_map && !normal_map) return fail("compute_normal_map is set but normal_map is null");
+  hipStream_t s = (hipStream_t)hip_stream;
+  const int64_t pix = (int64_t)B * h->C * h->W * h->H;
+  size_t old_keys = h->keys.n;
+  GVV_CHECK(h->cams.reserve((size_t)B * h->C));
+  GVV_CHECK(h->fnormal.reserve((size_t)B * h->F * 3));
+  GVV_CHECK(h->proj.reserve((size_t)B * h->C * h->N));
+  GVV_CHECK(h->keys.reserve((size_t)pix));
+  if (h->keys.n != old_keys || old_keys == 0)
+    GVV_CHECK(launch_fill_u64(h->keys.p, (int64_t)h->keys.n, kEmptyKey, s));
+  if (h->normal_map && (h->uvW != texW || h->uvH != texH)) {
+    std::vector<UvTexel> table;
+    build_uv_table(*h, texW, texH, table);
+    GVV_CHECK(h->d_uv_table.reserve(table.size()));
+    GVV_CHECK(hipMemcpyAsync(h->d_uv_table.p, table.data(), sizeof(UvTexel) * table.size(), hipMemcpyHostToDevice, s));
+    GVV_CHECK(hipStreamSynchronize(s));
+    h->uvW = texW; h->uvH = texH;
+  }
+  ForwardPlan p;
+  p.B = B; p.C = h->C; p.N = h->N; p.F = h->F;
+  p.albedo = h->albedo;
+  p.shade = (h->shading == kShaded && h->albedo != kNormal) || h->albedo == kLighting;
+  p.normal_map = h->normal_map != 0;
+  p.vpos = vpos; p.extr = extr; p.intr = intr;
+  p.faces = h->d_faces.p; p.csr_start = h->d_csr_start.p; p.csr_count = h->d_csr_count.p; p.csr_list = h->d_csr_list.p;
+  p.uv_table = h->d_uv_table.p;
+  p.cams = h->cams.p; p.fnormal = h->fnormal.p; p.proj = h->proj.p;
+  p.vnormal_out = vnormal;
+  p.normal_map_out = normal_map;
+  ResolveArgs& r = p.r;
+  r.B = B; r.C = h->C; r.N = h->N; r.W = h->W; r.H = h->H; r.texW = texW; r.texH = texH;
+  r.keys = h->keys.p; r.vpos = vpos; r.vcol = vcol; r.tex = tex; r.sh = sh; r.uv = h->d_uv.p;
+  r.faces = h->d_faces.p; r.cams = h->cams.p; r.vnormal = vnormal;
+  r.face = face; r.bary = bary; r.render = render;
+  p.prof = h->prof.enabled ? &h->prof : nullptr;
+  GVV_CHECK(launch_forward(p, s));
+  return 0;
+}
+
+int gvv_backward(void* handle, int32_t B, int32_t texH, int32_t texW, const float* g_render, const float* g_target,
+                 const float* target, const float* vpos, const float* vcol, const float* tex, const float* sh,
+                 const float* vnormal, const float* bary, const int32_t* face, const float* extr,
+                 const float* intr, float* g_pos, float* g_col, float* g_tex, float* g_sh, void* hip_stream) {
+  Handle* h = (Handle*)handle;
+  if (!h) return fail("null handle");
+  std::lock_guard<std::mutex> lock(h->mu);
+  if (check_device(h)) return 1;
+  if (B <= 0) return fail("batch size must be > 0");
+  if (texH <= 0 || texW <= 0) return fail("texture height/width must be > 0");
+  if (!g_pos || !g_col || !g_tex || !g_sh) return fail("null gradient output pointer");
+  hipStream_t s = (hipStream_t)hip_stream;
+  const int64_t nv = (int64_t)B * h->N * 3;
+  if (h->albedo == kNormal || h->albedo == kLighting) {
+    // the reference routes these modes to zero gradients (python/CudaRenderer.py:207-213)
+    GVV_CHECK(launch_fill_f32(g_pos, nv, 0.f, s));
+    GVV_CHECK(launch_fill_f32(g_col, nv, 0.f, s));
+    GVV_CHECK(launch_fill_f32(g_tex, (int64_t)B * texH * texW * 3, 0.f, s));
+    GVV_CHECK(launch_fill_f32(g_sh, (int64_t)B * h->C * 27, 0.f, s));
+    return 0;
+  }
+  if (!g_render || !vpos || !vcol || !tex || !sh || !vnormal || !face || !extr || !intr)
+    return fail("null tensor pointer");
+  if (g_target && !target) return fail("target_buffer_grad given without target_image");
+  const int tx = tiles(h->W), ty = tiles(h->H);
+  GVV_CHECK(h->cams.reserve((size_t)B * h->C));
+  GVV_CHECK(h->g_vn.reserve((size_t)nv));
+  GVV_CHECK(h->sh_partial.reserve((size_t)B * h->C * tx * ty * 27));
+  BackwardPlan p;
+  p.albedo = h->albedo;
+  p.shaded = h->shading == kShaded;
+  p.extr = extr; p.intr = intr; p.cams = h->cams.p;
+  p.prof = h->prof.enabled ? &h->prof : nullptr;
+  BackwardArgs& a = p.a;
+  a.B = B; a.C = h->C; a.N = h->N; a.F = h->F; a.W = h->W; a.H = h->H; a.texW = texW; a.texH = texH;
+  a.image_fs = h->image_fs; a.tiles_x = tx; a.tiles_y = ty;
+  a.g_render = g_render; a.g_target = g_target; a.target = target;
+  a.vpos = vpos; a.vcol = vcol; a.tex = tex; a.sh = sh; a.vnormal = vnormal; a.bary = bary; a.face = face;
+  a.faces = h->d_faces.p; a.uv = h->d_uv.p; a.cams = h->cams.p;
+  a.g_pos = g_pos; a.g_col = g_col; a.g_tex = g_tex; a.g_vn = h->g_vn.p; a.sh_partial = h->sh_partial.p; a.g_sh = g_sh;
+  GVV_CHECK(launch_backward(p, s));
+  return 0;
+}
+
+int gvv_profile_enable(void* handle, int32_t enable) {
+  Handle* h = (Handle*)handle;
+  if (!h) return fail("null handle");
+  std::lock_guard<std::mutex> lock(h->mu);
+  h->prof.reset();
+  h->prof.enabled = enable != 0;
+  return 0;
+}
+
+int64_t gvv_profile_report(void* handle, char* buf, int64_t cap) {
+  Handle* h = (Handle*)handle;
+  if (!h) return fail("null handle"), -1;
+  std::lock_guard<std::mutex> lock(h->mu);
+  h->prof.collect();
+  std::string out;
+  for (size_t k = 0; k < h->prof.names.size(); ++k) {
+    char line[256];
+    snprintf(line, sizeof(line), "%s %.6f %lld\n", h->prof.names[k].c_str(), h->prof.total_ms[k], h->prof.count[k]);
+    out += line;
+  }
+  if (buf && cap > 0) {
+    size_t n = std::min<size_t>(out.size(), (size_t)cap - 1);
+    memcpy(buf, out.data(), n);
+    buf[n] = 0;
+  }
+  return (int64_t)out.size();
+}
+
+}  // extern "C"

@@ -1,0 +1,149 @@
+// gvv_kernels.hpp -- launch plans shared by the forward/backward kernels and the C ABI.
+#pragma once
+#include <string>
+#include <vector>
+
+#include "gvv_math.hpp"
+
+namespace gvv {
+
+// Optional per-launch timing with HIP events on the launch stream.  When disabled, begin/end
+// do nothing.  collect() waits for the recorded events and accumulates milliseconds by name.
+struct Profiler {
+  bool enabled = false;
+  struct Mark { const char* name; hipEvent_t a, b; };
+  std::vector<Mark> marks;
+  std::vector<hipEvent_t> pool;
+  std::vector<std::string> names;
+  std::vector<double> total_ms;
+  std::vector<long long> count;
+
+  hipEvent_t take() {
+    if (!pool.empty()) { hipEvent_t e = pool.back(); pool.pop_back(); return e; }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+  }
+  void begin(const char* name, hipStream_t s) {
+    if (!enabled) return;
+    Mark m{name, take(), take()};
+    (void)hipEventRecord(m.a, s);
+    marks.push_back(m);
+  }
+  void end(hipStream_t s) {
+    if (!enabled || marks.empty()) return;
+    (void)hipEventRecord(marks.back().b, s);
+  }
+  void collect() {
+    for (auto& m : marks) {
+      (void)hipEventSynchronize(m.b);
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, m.a, m.b);
+      size_t k = 0;
+      while (k < names.size() && names[k] != m.name) ++k;
+      if (k == names.size()) { names.push_back(m.name); total_ms.push_back(0.0); count.push_back(0); }
+      total_ms[k] += ms; count[k] += 1;
+      pool.push_back(m.a); pool.push_back(m.b);
+    }
+    marks.clear();
+  }
+  void reset() { collect(); names.clear(); total_ms.clear(); count.clear(); }
+  ~Profiler() {
+    for (auto& m : marks) { (void)hipEventDestroy(m.a); (void)hipEventDestroy(m.b); }
+    for (auto e : pool) (void)hipEventDestroy(e);
+  }
+};
+
+struct ProfScope {
+  Profiler* p; hipStream_t s;
+  ProfScope(Profiler* p_, const char* name, hipStream_t s_) : p(p_), s(s_) { if (p) p->begin(name, s); }
+  ~ProfScope() { if (p) p->end(s); }
+};
+
+// Albedo and shading modes, numbered as the reference enums
+// (cpp/src/Renderer/CUDABasedRasterizationInput.h:25-35).
+enum Albedo { kVertexColor = 0, kTextured = 1, kNormal = 2, kLighting = 3, kForegroundMask = 4 };
+enum Shading { kShaded = 0, kShadeless = 1 };
+
+// One entry of the texel -> (face, barycentrics) table used by the normal map.
+struct UvTexel { int32_t face; float a, b, c; };
+
+struct ResolveArgs {
+  int B, C, N, W, H, texW, texH;
+  unsigned long long* keys;          // [B][C][H][W]; reset to empty after reading
+  const float* vpos;                 // [B][N][3]
+  const float* vcol;                 // [B][N][3]
+  const float* tex;                  // [B][texH][texW][3]
+  const float* sh;                   // [B][C][27]
+  const float* uv;                   // [F][6]
+  const int32_t* faces;              // [F][3]
+  const Camera* cams;                // [B][C]
+  const float* vnormal;              // [B][C][N][3]
+  int32_t* face;                     // [B][C][H][W]
+  float* bary;                       // [B][C][H][W][2]
+  float* render;                     // [B][C][H][W][3]
+};
+
+struct ForwardPlan {
+  int B, C, N, F;
+  int albedo;
+  bool shade;
+  bool normal_map;
+  const float* vpos;
+  const float* extr;
+  const float* intr;
+  const int32_t* faces;
+  const int32_t* csr_start;
+  const int32_t* csr_count;
+  const int32_t* csr_list;
+  const UvTexel* uv_table;
+  Camera* cams;
+  float* fnormal;                    // [B][F][3]
+  float4* proj;                      // [B][C][N]
+  float* vnormal_out;                // [B][C][N][3]
+  float* normal_map_out;             // [B][texH][texW][3]
+  ResolveArgs r;
+  Profiler* prof;
+};
+
+struct BackwardArgs {
+  int B, C, N, F, W, H, texW, texH, image_fs;
+  int tiles_x, tiles_y;
+  const float* g_render;             // [B][C][H][W][3]
+  const float* g_target;             // [B][C][H][W][3] or null (no model-to-data term)
+  const float* target;               // [B][C][H][W][3]
+  const float* vpos;
+  const float* vcol;
+  const float* tex;
+  const float* sh;
+  const float* vnormal;
+  const float* bary;                 // [B][C][H][W][2] or null (recomputed from the face id)
+  const int32_t* face;
+  const int32_t* faces;
+  const float* uv;
+  const Camera* cams;
+  float* g_pos;                      // [B][N][3]
+  float* g_col;                      // [B][N][3]
+  float* g_tex;                      // [B][texH][texW][3]
+  float* g_vn;                       // [B][N][3] scratch: dL/d(vertex normal)
+  float* sh_partial;                 // [B*C][tiles][27] scratch
+  float* g_sh;                       // [B][C][27]
+};
+
+struct BackwardPlan {
+  int albedo;
+  bool shaded;
+  BackwardArgs a;
+  const float* extr;
+  const float* intr;
+  Camera* cams;
+  Profiler* prof;
+};
+
+hipError_t launch_fill_u64(unsigned long long* p, int64_t n, unsigned long long v, hipStream_t s);
+hipError_t launch_fill_f32(float* p, int64_t n, float v, hipStream_t s);
+hipError_t launch_cameras(const float* extr, const float* intr, Camera* cams, int n, hipStream_t s);
+hipError_t launch_forward(const ForwardPlan& p, hipStream_t s);
+hipError_t launch_backward(const BackwardPlan& p, hipStream_t s);
+
+}  // namespace gvv

@@ -1,0 +1,54 @@
+"""ctypes binding of libgvv_amd.so (C ABI declared in include/gvv_renderer.h).
+
+The library is the only compute path: if it has not been built this module raises; there is
+no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.environ.get("GVV_AMD_LIB", os.path.join(_PKG, "libgvv_amd.so"))
+
+EXPORTS = ("gvv_last_error", "gvv_version", "gvv_create", "gvv_destroy", "gvv_forward", "gvv_backward",
+           "gvv_profile_enable", "gvv_profile_report")
+
+_lib = None
+
+
+class GvvError(RuntimeError):
+    pass
+
+
+def load():
+    """Load and type the HIP renderer library; raises if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise GvvError(f"{LIB_PATH} is missing: build it with `python gvv-differentiable-cuda-renderer_amd/build.py` "
+                       "(or __graft_entry__.build()); the renderer has no CPU fallback")
+    L = ctypes.CDLL(LIB_PATH)
+    P, i32, i64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64
+    L.gvv_last_error.restype = ctypes.c_char_p
+    L.gvv_version.restype = ctypes.c_int
+    L.gvv_create.argtypes = [P, i64, P, i64, i32, i32, i32, i32, i32, i32, i32, i32, i32, ctypes.POINTER(P)]
+    L.gvv_create.restype = ctypes.c_int
+    L.gvv_destroy.argtypes = [P]
+    L.gvv_destroy.restype = None
+    L.gvv_forward.argtypes = [P, i32, i32, i32] + [P] * 12
+    L.gvv_forward.restype = ctypes.c_int
+    L.gvv_backward.argtypes = [P, i32, i32, i32] + [P] * 17
+    L.gvv_backward.restype = ctypes.c_int
+    L.gvv_profile_enable.argtypes = [P, i32]
+    L.gvv_profile_enable.restype = ctypes.c_int
+    L.gvv_profile_report.argtypes = [P, ctypes.c_char_p, i64]
+    L.gvv_profile_report.restype = i64
+    _lib = L
+    return L
+
+
+def check(rc: int, what: str):
+    if rc != 0:
+        raise GvvError(f"{what}: {load().gvv_last_error().decode()}")

@@ -1,0 +1,102 @@
+/* gvv_renderer.h -- C ABI of the MI355X differentiable mesh rasterizer.
+ *
+ * Drop-in for the reference TensorFlow op pair (paths relative to the reference repository):
+ *   forward   REGISTER_OP("CudaRendererGpu")      cpp/src/TensorflowOperators/CudaRenderer/CudaRenderer.cpp:5-33
+ *             + its kernel CudaRenderer::Compute   CudaRenderer.cpp:298-335
+ *   backward  REGISTER_OP("CudaRendererGradGpu")  cpp/src/TensorflowOperators/CudaRenderer/CudaRendererGrad.cpp:6-39
+ *             + its kernel CudaRendererGrad::Compute CudaRendererGrad.cpp:252-293
+ *   Python    python/CudaRenderer.py:30-215 (mirrored by CudaRenderer.py in the package)
+ *
+ * Conventions
+ *   - All tensor arguments are borrowed, contiguous, row-major device pointers: float32 or
+ *     int32, in exactly the reference op layouts listed per function.
+ *   - Every function returns 0 on success and non-zero on error; gvv_last_error() then holds a
+ *     thread-local message.  Nothing calls exit().
+ *   - Work is enqueued on `hip_stream` (a hipStream_t; NULL = default stream).  A handle may be
+ *     used from one stream at a time; calls on one handle are serialised by a mutex.
+ *   - Albedo modes: 0 vertexColor, 1 textured, 2 normal, 3 lighting, 4 foregroundMask
+ *     (cpp/src/Renderer/CUDABasedRasterizationInput.h:25-28).  Shading: 0 shaded, 1 shadeless
+ *     (:33-35).  foregroundMask forces shadeless (CudaRenderer.cpp:72-76).
+ *   - Geometry is expected in millimetres: the reference's ray/triangle thresholds are
+ *     absolute (cpp/src/Utils/RendererUtil.h:29-47).
+ */
+#ifndef GVV_RENDERER_H
+#define GVV_RENDERER_H
+
+#include <stdint.h>
+
+#define GVV_RENDERER_ABI_VERSION 1
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Thread-local message of the last failed call on this thread. */
+const char* gvv_last_error(void);
+
+/* ABI version (GVV_RENDERER_ABI_VERSION). */
+int gvv_version(void);
+
+/* Creates a renderer for one mesh topology and attribute set.  Replaces the op attributes and
+ * the constructors CudaRenderer::CudaRenderer (CudaRenderer.cpp:37-155) and
+ * CUDABasedRasterization::CUDABasedRasterization (cpp/src/Renderer/CUDABasedRasterization.cpp:11-106).
+ *   faces        [3F] int32 vertex indices (attr `faces`)
+ *   texcoords    [6F] float, per face per corner (u, v) (attr `texture_coordinates`)
+ *   n_vertices, n_cameras, width (=render_resolution_u), height (=render_resolution_v),
+ *   albedo_mode, shading_mode, image_filter_size, texture_filter_size, compute_normal_map
+ * Host pointers; copied.  Rejects sizes <= 0, face indices out of range and texcoord counts
+ * other than 6F (the reference printed a warning and continued). */
+int gvv_create(const int32_t* faces, int64_t n_face_idx, const float* texcoords, int64_t n_texcoords,
+               int32_t n_vertices, int32_t n_cameras, int32_t width, int32_t height, int32_t albedo_mode,
+               int32_t shading_mode, int32_t image_filter_size, int32_t texture_filter_size,
+               int32_t compute_normal_map, void** handle);
+
+/* Releases a handle and its device buffers. */
+void gvv_destroy(void* handle);
+
+/* Forward render.  Replaces CudaRenderer::Compute (CudaRenderer.cpp:298-335) and
+ * renderBuffersGPU (cpp/src/Renderer/CUDABasedRasterization.cu:449-473).
+ *   inputs   vpos [B][N][3], vcol [B][N][3], tex [B][texH][texW][3], sh [B][C][27],
+ *            extr [B][C*12] (row-major 3x4 world->camera), intr [B][C*9] (row-major K)
+ *   outputs  bary [B][C][H][W][2], face [B][C][H][W] (int32, -1 = background),
+ *            render [B][C][H][W][3], vnormal [B][C][N][3],
+ *            normal_map [B][texH][texW][3] (written only when compute_normal_map; may be NULL
+ *            otherwise).
+ * The op's target_image_out output is the target input itself (an alias); it is not handled
+ * here.  Visibility ties between faces at the same integer depth resolve to the lowest face
+ * id (the reference resolves them by an unsynchronised last write). */
+int gvv_forward(void* handle, int32_t B, int32_t texH, int32_t texW, const float* vpos, const float* vcol,
+                const float* tex, const float* sh, const float* extr, const float* intr, float* bary,
+                int32_t* face, float* render, float* vnormal, float* normal_map, void* hip_stream);
+
+/* Backward.  Replaces CudaRendererGrad::Compute (CudaRendererGrad.cpp:252-293) and
+ * renderBuffersGradGPU (cpp/src/Renderer/CUDABasedRasterizationGrad.cu:624-635).
+ *   g_render  [B][C][H][W][3]  gradient of the render buffer (op input render_buffer_grad)
+ *   g_target  [B][C][H][W][3]  gradient of target_image_out (target_buffer_grad), or NULL for
+ *                              zero, which skips the model-to-data term
+ *   target    [B][C][H][W][3]  target image (required when g_target is given)
+ *   vpos, vcol, tex, sh, extr, intr as in gvv_forward; vnormal and face are forward outputs
+ *   bary      forward barycentrics, or NULL to recompute them from the face id (bit-identical
+ *             to the forward's values)
+ *   outputs   g_pos [B][N][3], g_col [B][N][3], g_tex [B][texH][texW][3], g_sh [B][C][27]
+ * In the normal and lighting modes every gradient is zero, as the reference's gradient
+ * routing returns (python/CudaRenderer.py:207-213). */
+int gvv_backward(void* handle, int32_t B, int32_t texH, int32_t texW, const float* g_render, const float* g_target,
+                 const float* target, const float* vpos, const float* vcol, const float* tex, const float* sh,
+                 const float* vnormal, const float* bary, const int32_t* face, const float* extr,
+                 const float* intr, float* g_pos, float* g_col, float* g_tex, float* g_sh, void* hip_stream);
+
+/* Kernel timing (no counterpart in the reference, which has no timing code).  When enabled,
+ * every kernel launch of this handle is bracketed by HIP events on its stream; enabling or
+ * disabling clears the accumulated figures. */
+int gvv_profile_enable(void* handle, int32_t enable);
+
+/* Waits for the recorded events and writes one line per kernel, "name total_ms launches\n",
+ * into buf (NUL-terminated, truncated to cap).  Returns the full report length, -1 on error. */
+int64_t gvv_profile_report(void* handle, char* buf, int64_t cap);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GVV_RENDERER_H */
