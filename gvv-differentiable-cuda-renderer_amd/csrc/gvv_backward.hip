@@ -3,12 +3,12 @@
 // Replaces renderBuffersGradGPU (reference cpp/src/Renderer/CUDABasedRasterizationGrad.cu:624-635)
 // and its per-pixel Jacobian chains (Grad.cu:114-617, RendererUtil.h:241-914).
 //
-//   k_pixel_grad   one 16x16 pixel tile per workgroup, one 8x8 quadrant per wavefront.
-//                  Each covered pixel forms its albedo, SH, barycentric-position,
-//                  model-to-data and normal terms in binary32.  Per-face terms are summed
-//                  across the lanes that share a face id (segmented wave reduction) before a
-//                  single lane issues the global atomics for the face's three vertices; SH
-//                  terms are summed per tile into a partial buffer.
+//   k_pixel_grad   one wavefront per 16x16 pixel tile (four 8x8 rounds).  Each covered pixel
+//                  forms its albedo, SH, barycentric-position, model-to-data and normal
+//                  terms.  Per-face terms are summed across the lanes that share a face id
+//                  (segmented DPP reduction) before a single lane issues the global atomics
+//                  for the face's three vertices; SH terms accumulate in registers over the
+//                  tile and are written once per tile into a partial buffer.
 //   k_sh_reduce    fixed-order reduction of the SH partials (deterministic).
 //   k_face_jac     the normal -> position chain.  Instead of walking every incident face of
 //                  every corner for every pixel (Grad.cu:559-615), the pixels accumulate
@@ -19,272 +19,315 @@
 
 namespace gvv {
 
-__device__ __forceinline__ float wave_sum(float v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
+// Sum of v over the 64 lanes, returned in every lane.  Inclusive DPP scan: row_shr 1/2/4/8
+// within each row of 16, then row_bcast 15 and 31 across rows; lane 63 ends with the total,
+// which readlane broadcasts.  All DPP steps are full-rate VALU adds.
+template <int CTRL, int ROW_MASK, bool BOUND_ZERO>
+__device__ __forceinline__ float dpp_mov(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xf, BOUND_ZERO));
 }
+
+__device__ __forceinline__ float wave_total(float v) {
+  v += dpp_mov<0x111, 0xf, true>(v);    // row_shr:1
+  v += dpp_mov<0x112, 0xf, true>(v);    // row_shr:2
+  v += dpp_mov<0x114, 0xf, true>(v);    // row_shr:4
+  v += dpp_mov<0x118, 0xf, true>(v);    // row_shr:8
+  v += dpp_mov<0x142, 0xa, false>(v);   // row_bcast:15 into rows 1 and 3
+  v += dpp_mov<0x143, 0xc, false>(v);   // row_bcast:31 into rows 2 and 3
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
+// Double-precision 3-vector for the barycentric Jacobian below.
+struct D3 { double x, y, z; };
+__device__ __forceinline__ D3 d3(double x, double y, double z) { D3 r; r.x = x; r.y = y; r.z = z; return r; }
+__device__ __forceinline__ D3 d3(V3 a) { return d3(a.x, a.y, a.z); }
+__device__ __forceinline__ D3 operator+(D3 a, D3 b) { return d3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ D3 operator-(D3 a, D3 b) { return d3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ D3 operator*(double s, D3 a) { return d3(s * a.x, s * a.y, s * a.z); }
+__device__ __forceinline__ double dot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ D3 cross(D3 a, D3 b) { return d3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
 
 // d(a,b,c)/d(v0,v1,v2) for a fixed ray (reference dJBCDVerpos, RendererUtil.h:670-861),
 // contracted with the row vector gb (gradient w.r.t. the three barycentrics).  Exploits that
 // dN/d(v_vtx . e_k) = e_k x A_vtx with A_0 = v1-v2, A_1 = v2-v0, A_2 = v0-v1, and that the
-// derivatives of the edges and of the vertices are unit vectors.  Returns the 9-vector
-// gradient w.r.t. (v0, v1, v2) in `out` (added).
-__device__ __forceinline__ void bary_position_grad(V3 o, V3 d, V3 v0, V3 v1, V3 v2, V3 gb, float out[9]) {
-  V3 N = cross(v1 - v0, v2 - v0);
-  float den = dot(N, N);
-  if (fabsf(dot(normalize(d), normalize(N))) < 0.001f || fabsf(den * den) < 0.001f) return;
-  float nd = dot(N, d);
-  V3 w = v0 - o;
-  float wN = dot(w, N);
-  float t = wN / nd;                        // ray parameter of the plane hit: (v0 - o).N / N.d
-  V3 P = o + t * d;
-  V3 E[2] = {v2 - v1, v0 - v2};
-  V3 vp[2] = {P - v1, P - v2};
-  V3 Cr[2] = {cross(E[0], vp[0]), cross(E[1], vp[1])};
-  float NC[2] = {dot(N, Cr[0]), dot(N, Cr[1])};
-  V3 vpxN[2] = {cross(vp[0], N), cross(vp[1], N)};
-  V3 NxE[2] = {cross(N, E[0]), cross(N, E[1])};
-  float NEd[2] = {dot(N, cross(E[0], d)), dot(N, cross(E[1], d))};
+// derivatives of the edges and of the vertices are unit vectors.  Adds the 9-vector gradient
+// w.r.t. (v0, v1, v2) to `out`.
+//
+// The early exit is the reference's binary32 test.  The Jacobian itself is evaluated in FP64:
+// it scales like 1/(N.d) and several of its terms cancel near grazing incidence, so binary32
+// loses ~eps/cos relative accuracy there (measured 2.8e-4 max-norm on a 5k-face sphere, where
+// FP64 on binary32 inputs matches the FP64 oracle to ~1e-7).  It is ~150 flops per covered pixel.
+__device__ __forceinline__ void bary_position_grad(V3 of, V3 df, V3 v0f, V3 v1f, V3 v2f, V3 gbf, float out[9]) {
+  {
+    V3 Nf = cross(v1f - v0f, v2f - v0f);
+    float denf = dot(Nf, Nf);
+    if (fabsf(dot(normalize(df), normalize(Nf))) < 0.001f || fabsf(denf * denf) < 0.001f) return;
+  }
+  const D3 o = d3(of), d = d3(df), v0 = d3(v0f), v1 = d3(v1f), v2 = d3(v2f);
+  const D3 N = cross(v1 - v0, v2 - v0);
+  const double den = dot(N, N);
+  const double nd = dot(N, d);
+  const D3 w = v0 - o;
+  const double wN = dot(w, N);
+  const double t = wN / nd;                 // ray parameter of the plane hit: (v0 - o).N / N.d
+  const D3 P = o + t * d;
+  const D3 E[2] = {v2 - v1, v0 - v2};
+  const D3 vp[2] = {P - v1, P - v2};
+  const D3 Cr[2] = {cross(E[0], vp[0]), cross(E[1], vp[1])};
+  const double NC[2] = {dot(N, Cr[0]), dot(N, Cr[1])};
+  const D3 vpxN[2] = {cross(vp[0], N), cross(vp[1], N)};
+  const D3 NxE[2] = {cross(N, E[0]), cross(N, E[1])};
+  const double NEd[2] = {dot(N, cross(E[0], d)), dot(N, cross(E[1], d))};
   // combine the two independent rows: J[2] = -J[0] - J[1]
-  float w0 = gb.x - gb.z, w1 = gb.y - gb.z;
-  float inv_den = 1.f / den, inv_den2 = inv_den * inv_den, inv_nd = 1.f / nd;
-  float c_dt = -wN * inv_nd * inv_nd;
-  V3 A[3] = {v1 - v2, v2 - v0, v0 - v1};
+  const double w0 = (double)gbf.x - (double)gbf.z, w1 = (double)gbf.y - (double)gbf.z;
+  const double inv_den = 1.0 / den, inv_den2 = inv_den * inv_den, inv_nd = 1.0 / nd;
+  const double c_dt = -wN * inv_nd * inv_nd;
+  const D3 A[3] = {v1 - v2, v2 - v0, v0 - v1};
   // sign of dE_r / dV_vtx: E_0 = v2 - v1, E_1 = v0 - v2
-  const float sE[2][3] = {{0.f, -1.f, 1.f}, {1.f, 0.f, -1.f}};
+  const double sE[2][3] = {{0.0, -1.0, 1.0}, {1.0, 0.0, -1.0}};
 #pragma unroll
   for (int vtx = 0; vtx < 3; ++vtx) {
-    V3 Axd = cross(A[vtx], d), Axw = cross(A[vtx], w), AxN = cross(A[vtx], N);
-    V3 AxC0 = cross(A[vtx], Cr[0]), AxC1 = cross(A[vtx], Cr[1]);
-    float axd[3] = {Axd.x, Axd.y, Axd.z}, axw[3] = {Axw.x, Axw.y, Axw.z}, axn[3] = {AxN.x, AxN.y, AxN.z};
-    float axc[2][3] = {{AxC0.x, AxC0.y, AxC0.z}, {AxC1.x, AxC1.y, AxC1.z}};
-    float nk[3] = {N.x, N.y, N.z};
-    float vpn[2][3] = {{vpxN[0].x, vpxN[0].y, vpxN[0].z}, {vpxN[1].x, vpxN[1].y, vpxN[1].z}};
-    float nxe[2][3] = {{NxE[0].x, NxE[0].y, NxE[0].z}, {NxE[1].x, NxE[1].y, NxE[1].z}};
+    const D3 Axd = cross(A[vtx], d), Axw = cross(A[vtx], w), AxN = cross(A[vtx], N);
+    const D3 AxC0 = cross(A[vtx], Cr[0]), AxC1 = cross(A[vtx], Cr[1]);
+    const double axd[3] = {Axd.x, Axd.y, Axd.z}, axw[3] = {Axw.x, Axw.y, Axw.z}, axn[3] = {AxN.x, AxN.y, AxN.z};
+    const double axc[2][3] = {{AxC0.x, AxC0.y, AxC0.z}, {AxC1.x, AxC1.y, AxC1.z}};
+    const double nk[3] = {N.x, N.y, N.z};
+    const double vpn[2][3] = {{vpxN[0].x, vpxN[0].y, vpxN[0].z}, {vpxN[1].x, vpxN[1].y, vpxN[1].z}};
+    const double nxe[2][3] = {{NxE[0].x, NxE[0].y, NxE[0].z}, {NxE[1].x, NxE[1].y, NxE[1].z}};
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      float dt = c_dt * axd[k] + inv_nd * ((vtx == 0 ? nk[k] : 0.f) + axw[k]);
-      float NdN2 = 2.f * axn[k];
-      float J[2];
+      const double dt = c_dt * axd[k] + inv_nd * ((vtx == 0 ? nk[k] : 0.0) + axw[k]);
+      const double NdN2 = 2.0 * axn[k];
+      double J[2];
 #pragma unroll
       for (int r = 0; r < 2; ++r) {
-        float dtmp = axc[r][k] + sE[r][vtx] * vpn[r][k] + dt * NEd[r] - (vtx == r + 1 ? nxe[r][k] : 0.f);
+        const double dtmp = axc[r][k] + sE[r][vtx] * vpn[r][k] + dt * NEd[r] - (vtx == r + 1 ? nxe[r][k] : 0.0);
         J[r] = dtmp * inv_den - NC[r] * inv_den2 * NdN2;
       }
-      out[3 * vtx + k] += w0 * J[0] + w1 * J[1];
+      out[3 * vtx + k] += (float)(w0 * J[0] + w1 * J[1]);
     }
   }
 }
 
+// Per-pixel gradient terms of one covered pixel (reference Grad.cu:205-555).  Adds the SH
+// terms to sh_acc and the texture terms to g_tex directly; returns the per-face terms
+// (position, vertex-normal gradient and vertex-colour rows for the three corners).
 template <int ALBEDO, bool SHADED, bool M2D, bool HAVE_BARY>
-__global__ void __launch_bounds__(256) k_pixel_grad(BackwardArgs a) {
+__device__ __forceinline__ void pixel_terms(const BackwardArgs& a, int bc, int b, int px, int py, size_t idx, int idf,
+                                            float g_pos[9], float g_vn[9], float g_col[9], float sh_acc[27]) {
+  const Camera& cam = a.cams[bc];
+  const V3 o = cam_origin(cam);
+  const V3 d = pixel_ray(cam, px + 0.5f, py + 0.5f);
+  const int i0 = a.faces[3 * idf], i1 = a.faces[3 * idf + 1], i2 = a.faces[3 * idf + 2];
+  const float* vp = a.vpos + (size_t)b * a.N * 3;
+  const V3 p0 = ld3(vp + 3 * i0), p1 = ld3(vp + 3 * i1), p2 = ld3(vp + 3 * i2);
+  V3 abc;
+  if (HAVE_BARY) {
+    const float ba = a.bary[2 * idx], bb = a.bary[2 * idx + 1];
+    abc = v3(ba, bb, 1.f - ba - bb);
+  } else {
+    abc = ray_barycentric(tri_setup(p0, p1, p2), o / 1000.f, d);
+  }
+  const float w3[3] = {abc.x, abc.y, abc.z};
+  const float* vnb = a.vnormal + (size_t)bc * a.N * 3;
+  const V3 n0 = ld3(vnb + 3 * i0), n1 = ld3(vnb + 3 * i1), n2 = ld3(vnb + 3 * i2);
+  const V3 un = n0 * abc.x + n1 * abc.y + n2 * abc.z;
+  const float unl = length(un);
+  V3 n = un / unl;
+  const bool flipped = dot(n, d) > 0.f;
+  if (flipped) n = -n;
+  const float* shc = a.sh + (size_t)bc * 27;
+  const float* gr = a.g_render + 3 * idx;
+  const V3 g = v3(gr[0], gr[1], gr[2]);
+
+  V3 L = v3(1.f, 1.f, 1.f);
+  if (SHADED) L = sh_illum(n, shc);
+
+  V3 alb = v3(0.f, 0.f, 0.f);
+  if (ALBEDO == kVertexColor) {
+    const float* vc = a.vcol + (size_t)b * a.N * 3;
+    const V3 c0 = ld3(vc + 3 * i0), c1 = ld3(vc + 3 * i1), c2 = ld3(vc + 3 * i2);
+    alb = c0 * abc.x + c1 * abc.y + c2 * abc.z;
+    const V3 ga = v3(g.x * L.x, g.y * L.y, g.z * L.z);
+#pragma unroll
+    for (int r = 0; r < 3; ++r) { g_col[3 * r] = ga.x * w3[r]; g_col[3 * r + 1] = ga.y * w3[r]; g_col[3 * r + 2] = ga.z * w3[r]; }
+  } else if (ALBEDO == kTextured) {
+    // the backward's albedo is the reference's four-texel blend (Grad.cu:282-312), not the
+    // forward's nearest texel
+    const TexCoord tc = tex_coord(a.uv + 6 * (size_t)idf, abc, a.texW, a.texH);
+    const float* tb = a.tex + (size_t)b * a.texH * a.texW * 3;
+    const float LU = tc.LU + 0.5f, HU = tc.LU + 1.5f, LV = tc.LV + 0.5f, HV = tc.LV + 1.5f;
+    const int lu = min(max(tc.LU, 0), a.texW - 1), hu = min(max(tc.LU + 1, 0), a.texW - 1);
+    const int lv = min(max(tc.LV, 0), a.texH - 1), hv = min(max(tc.LV + 1, 0), a.texH - 1);
+    const V3 cLL = ld3(tb + 3 * ((size_t)lv * a.texW + lu)), cHL = ld3(tb + 3 * ((size_t)lv * a.texW + hu));
+    const V3 cLH = ld3(tb + 3 * ((size_t)hv * a.texW + lu)), cHH = ld3(tb + 3 * ((size_t)hv * a.texW + hu));
+    const float U0 = tc.U0, V0 = tc.V0;
+    alb = (V0 - LV) * (((U0 - LU) * cLL) + ((HU - U0) * cHL)) + (HV - V0) * (((U0 - LU) * cLH) + ((HU - U0) * cHH));
+    if (!flipped && tc.LV >= 0 && tc.LV < a.texH && tc.LU >= 0 && tc.LU < a.texW) {
+      float* gt = a.g_tex + (size_t)b * a.texH * a.texW * 3 + 3 * ((size_t)tc.LV * a.texW + tc.LU);
+      atomicAdd(gt + 0, g.x * L.x);
+      atomicAdd(gt + 1, g.y * L.y);
+      atomicAdd(gt + 2, g.z * L.z);
+    }
+  }
+
+  if (SHADED) {
+    const V3 gl = v3(g.x * alb.x, g.y * alb.y, g.z * alb.z);
+    const float Y[9] = {1.f, n.y, n.z, n.x, n.x * n.y, n.z * n.y, 3.f * n.z * n.z - 1.f, n.x * n.z,
+                        n.x * n.x - n.y * n.y};
+    const float glc[3] = {gl.x, gl.y, gl.z};
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch)
+#pragma unroll
+      for (int k = 0; k < 9; ++k) sh_acc[ch * 9 + k] += glc[ch] * Y[k];
+
+    // d light / d n (RendererUtil.h:371-391), then through the normalisation of the
+    // unflipped interpolated normal (RendererUtil.h:398-415).
+    float gn[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const float* sc = shc + 9 * i;
+      gn[0] += glc[i] * (sc[3] + sc[4] * n.y + sc[7] * n.z + sc[8] * 2.f * n.x);
+      gn[1] += glc[i] * (sc[1] + sc[4] * n.x + sc[5] * n.z + sc[8] * -2.f * n.y);
+      gn[2] += glc[i] * (sc[2] + sc[5] * n.y + sc[6] * 6.f * n.z + sc[7] * n.x);
+    }
+    const float unl2 = unl * unl, inv_unl3 = 1.f / (unl2 * unl);
+    const float gnu = gn[0] * un.x + gn[1] * un.y + gn[2] * un.z;
+    const V3 gu = v3((gn[0] * unl2 - un.x * gnu) * inv_unl3, (gn[1] * unl2 - un.y * gnu) * inv_unl3,
+                     (gn[2] * unl2 - un.z * gnu) * inv_unl3);
+#pragma unroll
+    for (int r = 0; r < 3; ++r) { g_vn[3 * r] = gu.x * w3[r]; g_vn[3 * r + 1] = gu.y * w3[r]; g_vn[3 * r + 2] = gu.z * w3[r]; }
+    const V3 gb = v3(dot(gu, n0), dot(gu, n1), dot(gu, n2));
+    bary_position_grad(o, d, p0, p1, p2, gb, g_pos);
+  }
+
+  if (M2D) {
+    // Model-to-data term: target image gradient at integer pixel coordinates
+    // (RendererUtil.h:566-620) times the projection Jacobian of the interpolated surface
+    // point (RendererUtil.h:275-332).
+    const float* gtp = a.g_target + 3 * idx;
+    const V3 gT = v3(gtp[0], gtp[1], gtp[2]);
+    const int fs = a.image_fs;
+    float q0 = 0.f, q1 = 0.f;
+    if (px >= fs + 1 && py >= fs + 1 && px < a.W - (fs + 1) && py < a.H - (fs + 1)) {
+      const float* img = a.target + (size_t)bc * a.W * a.H * 3;
+      V3 du = v3(0.f, 0.f, 0.f), dv = v3(0.f, 0.f, 0.f);
+      float norm = 0.f;
+      for (int y = -fs; y <= fs; ++y)
+        for (int x = -fs; x <= fs; ++x) {
+          const float den = (float)(x * x + y * y);
+          float gu_ = 0.f, gv_ = 0.f;
+          if (den != 0.f) { gu_ = (float)x / den; gv_ = (float)y / den; }
+          const V3 I = ld3(img + 3 * ((size_t)(py + y) * a.W + (px + x)));
+          du = du + I * gu_;
+          dv = dv + I * gv_;
+          norm += fabsf(gu_);
+        }
+      q0 = dot(gT, du) / norm;
+      q1 = dot(gT, dv) / norm;
+    }
+    if (q0 != 0.f || q1 != 0.f) {
+      const V3 X = p0 * abc.x + p1 * abc.y + p2 * abc.z;
+      const float* K = cam.K;
+      const float* E = cam.E;
+      float KE[3][4];
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int cc = 0; cc < 4; ++cc) KE[r][cc] = K[r * 3] * E[cc] + K[r * 3 + 1] * E[4 + cc] + K[r * 3 + 2] * E[8 + cc];
+      float P[3];
+#pragma unroll
+      for (int r = 0; r < 3; ++r) P[r] = KE[r][0] * X.x + KE[r][1] * X.y + KE[r][2] * X.z + KE[r][3];
+      if (fabsf(P[2]) > 0.0001f) {
+        const float iz = 1.f / P[2];
+        const float dd0[3] = {iz, 0.f, -P[0] * iz * iz}, dd1[3] = {0.f, iz, -P[1] * iz * iz};
+        float gx[3];
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) {
+          const float j0 = dd0[0] * KE[0][cc] + dd0[2] * KE[2][cc];
+          const float j1 = dd1[1] * KE[1][cc] + dd1[2] * KE[2][cc];
+          gx[cc] = q0 * j0 + q1 * j1;
+        }
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+          g_pos[3 * r] += gx[0] * w3[r]; g_pos[3 * r + 1] += gx[1] * w3[r]; g_pos[3 * r + 2] += gx[2] * w3[r];
+        }
+      }
+    }
+  }
+}
+
+// One wavefront per 16x16 pixel tile of one (b, c) image, in four 8x8 rounds.  Per-face terms
+// are summed over the lanes that share a face id (segmented DPP reduction) and one lane issues
+// the atomics for that face's three vertices; SH terms accumulate in registers over the whole
+// tile and are reduced once into the tile's partial slot.
+template <int ALBEDO, bool SHADED, bool M2D, bool HAVE_BARY>
+__global__ void __launch_bounds__(64) k_pixel_grad(BackwardArgs a) {
   constexpr bool NEED_POS = SHADED || M2D;
   constexpr bool NEED_COL = ALBEDO == kVertexColor;
-  __shared__ float sh_lds[4][27];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x;
   const int tile = blockIdx.x, bc = blockIdx.y;
   const int b = bc / a.C;
   const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
-  const int px = tx * 16 + (wave & 1) * 8 + (lane & 7);
-  const int py = ty * 16 + (wave >> 1) * 8 + (lane >> 3);
-  const bool valid = px < a.W && py < a.H;
-  const size_t idx = ((size_t)bc * a.H + (valid ? py : 0)) * a.W + (valid ? px : 0);
-  int idf = valid ? a.face[idx] : -1;
-  if (idf >= a.F) idf = -1;   // ids outside the mesh are treated as background
-
-  float g_pos[9], g_vn[9], g_col[9], g_sh[27];
+  float sh_acc[27];
 #pragma unroll
-  for (int k = 0; k < 9; ++k) { g_pos[k] = 0.f; g_vn[k] = 0.f; g_col[k] = 0.f; }
-#pragma unroll
-  for (int k = 0; k < 27; ++k) g_sh[k] = 0.f;
+  for (int k = 0; k < 27; ++k) sh_acc[k] = 0.f;
+  bool any = false;
 
-  if (idf >= 0) {
-    const Camera& cam = a.cams[bc];
-    const V3 o = cam_origin(cam);
-    const V3 d = pixel_ray(cam, px + 0.5f, py + 0.5f);
-    const int i0 = a.faces[3 * idf], i1 = a.faces[3 * idf + 1], i2 = a.faces[3 * idf + 2];
-    const float* vp = a.vpos + (size_t)b * a.N * 3;
-    const V3 p0 = ld3(vp + 3 * i0), p1 = ld3(vp + 3 * i1), p2 = ld3(vp + 3 * i2);
-    V3 abc;
-    if (HAVE_BARY) {
-      float ba = a.bary[2 * idx], bb = a.bary[2 * idx + 1];
-      abc = v3(ba, bb, 1.f - ba - bb);
-    } else {
-      abc = ray_barycentric(tri_setup(p0, p1, p2), o / 1000.f, d);
-    }
-    const float w3[3] = {abc.x, abc.y, abc.z};
-    const float* vnb = a.vnormal + (size_t)bc * a.N * 3;
-    const V3 n0 = ld3(vnb + 3 * i0), n1 = ld3(vnb + 3 * i1), n2 = ld3(vnb + 3 * i2);
-    const V3 un = n0 * abc.x + n1 * abc.y + n2 * abc.z;
-    const float unl = length(un);
-    V3 n = un / unl;
-    const bool flipped = dot(n, d) > 0.f;
-    if (flipped) n = -n;
-    const float* shc = a.sh + (size_t)bc * 27;
-    const float* gr = a.g_render + 3 * idx;
-    const V3 g = v3(gr[0], gr[1], gr[2]);
-
-    V3 L = v3(1.f, 1.f, 1.f);
-    if (SHADED) L = sh_illum(n, shc);
-
-    V3 alb = v3(0.f, 0.f, 0.f);
-    if (ALBEDO == kVertexColor) {
-      const float* vc = a.vcol + (size_t)b * a.N * 3;
-      V3 c0 = ld3(vc + 3 * i0), c1 = ld3(vc + 3 * i1), c2 = ld3(vc + 3 * i2);
-      alb = c0 * abc.x + c1 * abc.y + c2 * abc.z;
-      V3 ga = v3(g.x * L.x, g.y * L.y, g.z * L.z);
+  for (int round = 0; round < 4; ++round) {
+    const int px = tx * 16 + (round & 1) * 8 + (lane & 7);
+    const int py = ty * 16 + (round >> 1) * 8 + (lane >> 3);
+    const bool valid = px < a.W && py < a.H;
+    const size_t idx = ((size_t)bc * a.H + (valid ? py : 0)) * a.W + (valid ? px : 0);
+    int idf = valid ? a.face[idx] : -1;
+    if (idf >= a.F) idf = -1;   // ids outside the mesh are treated as background
+    uint64_t pending = __ballot(idf >= 0);
+    if (!pending) continue;
+    any = true;
+    float g_pos[9], g_vn[9], g_col[9];
 #pragma unroll
-      for (int r = 0; r < 3; ++r) { g_col[3 * r] = ga.x * w3[r]; g_col[3 * r + 1] = ga.y * w3[r]; g_col[3 * r + 2] = ga.z * w3[r]; }
-    } else if (ALBEDO == kTextured) {
-      TexCoord tc = tex_coord(a.uv + 6 * (size_t)idf, abc, a.texW, a.texH);
-      const float* tb = a.tex + (size_t)b * a.texH * a.texW * 3;
-      const float LU = tc.LU + 0.5f, HU = tc.LU + 1.5f, LV = tc.LV + 0.5f, HV = tc.LV + 1.5f;
-      const int lu = min(max(tc.LU, 0), a.texW - 1), hu = min(max(tc.LU + 1, 0), a.texW - 1);
-      const int lv = min(max(tc.LV, 0), a.texH - 1), hv = min(max(tc.LV + 1, 0), a.texH - 1);
-      V3 cLL = ld3(tb + 3 * ((size_t)lv * a.texW + lu)), cHL = ld3(tb + 3 * ((size_t)lv * a.texW + hu));
-      V3 cLH = ld3(tb + 3 * ((size_t)hv * a.texW + lu)), cHH = ld3(tb + 3 * ((size_t)hv * a.texW + hu));
-      const float U0 = tc.U0, V0 = tc.V0;
-      alb = (V0 - LV) * (((U0 - LU) * cLL) + ((HU - U0) * cHL)) + (HV - V0) * (((U0 - LU) * cLH) + ((HU - U0) * cHH));
-      if (!flipped && tc.LV >= 0 && tc.LV < a.texH && tc.LU >= 0 && tc.LU < a.texW) {
-        float* gt = a.g_tex + (size_t)b * a.texH * a.texW * 3 + 3 * ((size_t)tc.LV * a.texW + tc.LU);
-        atomicAdd(gt + 0, g.x * L.x);
-        atomicAdd(gt + 1, g.y * L.y);
-        atomicAdd(gt + 2, g.z * L.z);
-      }
-    }
+    for (int k = 0; k < 9; ++k) { g_pos[k] = 0.f; g_vn[k] = 0.f; g_col[k] = 0.f; }
+    if (idf >= 0) pixel_terms<ALBEDO, SHADED, M2D, HAVE_BARY>(a, bc, b, px, py, idx, idf, g_pos, g_vn, g_col, sh_acc);
 
-    if (SHADED) {
-      const V3 gl = v3(g.x * alb.x, g.y * alb.y, g.z * alb.z);
-      const float Y[9] = {1.f, n.y, n.z, n.x, n.x * n.y, n.z * n.y, 3.f * n.z * n.z - 1.f, n.x * n.z,
-                          n.x * n.x - n.y * n.y};
-      const float glc[3] = {gl.x, gl.y, gl.z};
+    if (NEED_POS || NEED_COL) {
+      while (pending) {
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const int f = __builtin_amdgcn_readlane(idf, leader);
+        const bool mine = idf == f;
+        pending &= ~__ballot(mine);
+        const int v0 = a.faces[3 * f], v1 = a.faces[3 * f + 1], v2 = a.faces[3 * f + 2];
+        const int vid[3] = {v0, v1, v2};
+        if (NEED_POS) {
 #pragma unroll
-      for (int ch = 0; ch < 3; ++ch)
-#pragma unroll
-        for (int k = 0; k < 9; ++k) g_sh[ch * 9 + k] = glc[ch] * Y[k];
-
-      // d light / d n (RendererUtil.h:371-391), then through the normalisation of the
-      // unflipped interpolated normal (RendererUtil.h:398-415).
-      float gn[3] = {0.f, 0.f, 0.f};
-#pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        const float* s = shc + 9 * i;
-        gn[0] += glc[i] * (s[3] + s[4] * n.y + s[7] * n.z + s[8] * 2.f * n.x);
-        gn[1] += glc[i] * (s[1] + s[4] * n.x + s[5] * n.z + s[8] * -2.f * n.y);
-        gn[2] += glc[i] * (s[2] + s[5] * n.y + s[6] * 6.f * n.z + s[7] * n.x);
-      }
-      const float unl2 = unl * unl, inv_unl3 = 1.f / (unl2 * unl);
-      const float gnu = gn[0] * un.x + gn[1] * un.y + gn[2] * un.z;
-      const V3 gu = v3((gn[0] * unl2 - un.x * gnu) * inv_unl3, (gn[1] * unl2 - un.y * gnu) * inv_unl3,
-                       (gn[2] * unl2 - un.z * gnu) * inv_unl3);
-#pragma unroll
-      for (int r = 0; r < 3; ++r) { g_vn[3 * r] = gu.x * w3[r]; g_vn[3 * r + 1] = gu.y * w3[r]; g_vn[3 * r + 2] = gu.z * w3[r]; }
-      const V3 gb = v3(dot(gu, n0), dot(gu, n1), dot(gu, n2));
-      bary_position_grad(o, d, p0, p1, p2, gb, g_pos);
-    }
-
-    if (M2D) {
-      // Model-to-data term: target image gradient at integer pixel coordinates
-      // (RendererUtil.h:566-620) times the projection Jacobian of the interpolated surface
-      // point (RendererUtil.h:275-332).
-      const float* gtp = a.g_target + 3 * idx;
-      const V3 gT = v3(gtp[0], gtp[1], gtp[2]);
-      const int fs = a.image_fs;
-      float q0 = 0.f, q1 = 0.f;
-      if (px >= fs + 1 && py >= fs + 1 && px < a.W - (fs + 1) && py < a.H - (fs + 1)) {
-        const float* img = a.target + (size_t)bc * a.W * a.H * 3;
-        V3 du = v3(0.f, 0.f, 0.f), dv = v3(0.f, 0.f, 0.f);
-        float norm = 0.f;
-        for (int y = -fs; y <= fs; ++y)
-          for (int x = -fs; x <= fs; ++x) {
-            float den = (float)(x * x + y * y);
-            float gu_ = 0.f, gv_ = 0.f;
-            if (den != 0.f) { gu_ = (float)x / den; gv_ = (float)y / den; }
-            V3 I = ld3(img + 3 * ((size_t)(py + y) * a.W + (px + x)));
-            du = du + I * gu_;
-            dv = dv + I * gv_;
-            norm += fabsf(gu_);
+          for (int k = 0; k < 9; ++k) {
+            const float s = wave_total(mine ? g_pos[k] : 0.f);
+            if (lane == leader && s != 0.f) atomicAdd(a.g_pos + ((size_t)b * a.N + vid[k / 3]) * 3 + k % 3, s);
           }
-        q0 = dot(gT, du) / norm;
-        q1 = dot(gT, dv) / norm;
-      }
-      if (q0 != 0.f || q1 != 0.f) {
-        const V3 X = p0 * abc.x + p1 * abc.y + p2 * abc.z;
-        const float* K = cam.K;
-        const float* E = cam.E;
-        float KE[3][4];
+        }
+        if (SHADED) {
 #pragma unroll
-        for (int r = 0; r < 3; ++r)
-#pragma unroll
-          for (int cc = 0; cc < 4; ++cc) KE[r][cc] = K[r * 3] * E[cc] + K[r * 3 + 1] * E[4 + cc] + K[r * 3 + 2] * E[8 + cc];
-        float P[3];
-#pragma unroll
-        for (int r = 0; r < 3; ++r) P[r] = KE[r][0] * X.x + KE[r][1] * X.y + KE[r][2] * X.z + KE[r][3];
-        if (fabsf(P[2]) > 0.0001f) {
-          const float iz = 1.f / P[2];
-          const float dd0[3] = {iz, 0.f, -P[0] * iz * iz}, dd1[3] = {0.f, iz, -P[1] * iz * iz};
-          float gx[3];
-#pragma unroll
-          for (int cc = 0; cc < 3; ++cc) {
-            float j0 = dd0[0] * KE[0][cc] + dd0[2] * KE[2][cc];
-            float j1 = dd1[1] * KE[1][cc] + dd1[2] * KE[2][cc];
-            gx[cc] = q0 * j0 + q1 * j1;
+          for (int k = 0; k < 9; ++k) {
+            const float s = wave_total(mine ? g_vn[k] : 0.f);
+            if (lane == leader && s != 0.f) atomicAdd(a.g_vn + ((size_t)b * a.N + vid[k / 3]) * 3 + k % 3, s);
           }
+        }
+        if (NEED_COL) {
 #pragma unroll
-          for (int r = 0; r < 3; ++r) {
-            g_pos[3 * r] += gx[0] * w3[r]; g_pos[3 * r + 1] += gx[1] * w3[r]; g_pos[3 * r + 2] += gx[2] * w3[r];
+          for (int k = 0; k < 9; ++k) {
+            const float s = wave_total(mine ? g_col[k] : 0.f);
+            if (lane == leader && s != 0.f) atomicAdd(a.g_col + ((size_t)b * a.N + vid[k / 3]) * 3 + k % 3, s);
           }
         }
       }
     }
   }
 
-  // ---- SH: tile partial sums (every lane participates) ----
   if (SHADED) {
+    float* out = a.sh_partial + ((size_t)bc * gridDim.x + tile) * 27;
 #pragma unroll
     for (int k = 0; k < 27; ++k) {
-      float s = wave_sum(g_sh[k]);
-      if (lane == 0) sh_lds[wave][k] = s;
-    }
-    __syncthreads();
-    if (threadIdx.x < 27) {
-      float s = sh_lds[0][threadIdx.x] + sh_lds[1][threadIdx.x] + sh_lds[2][threadIdx.x] + sh_lds[3][threadIdx.x];
-      a.sh_partial[((size_t)bc * gridDim.x + tile) * 27 + threadIdx.x] = s;
-    }
-  }
-
-  // ---- per-face segmented reduction, one lane issues the atomics ----
-  if (NEED_POS || NEED_COL) {
-    uint64_t pending = __ballot(idf >= 0);
-    while (pending) {
-      const int leader = __ffsll((unsigned long long)pending) - 1;
-      const int f = __shfl(idf, leader, 64);
-      const bool mine = idf == f;
-      pending &= ~__ballot(mine);
-      int vid[3] = {0, 0, 0};
-      if (lane == leader) { vid[0] = a.faces[3 * f]; vid[1] = a.faces[3 * f + 1]; vid[2] = a.faces[3 * f + 2]; }
-      if (NEED_POS) {
-#pragma unroll
-        for (int k = 0; k < 9; ++k) {
-          float s = wave_sum(mine ? g_pos[k] : 0.f);
-          if (lane == leader && s != 0.f) atomicAdd(a.g_pos + ((size_t)b * a.N + vid[k / 3]) * 3 + k % 3, s);
-        }
-      }
-      if (SHADED) {
-#pragma unroll
-        for (int k = 0; k < 9; ++k) {
-          float s = wave_sum(mine ? g_vn[k] : 0.f);
-          if (lane == leader && s != 0.f) atomicAdd(a.g_vn + ((size_t)b * a.N + vid[k / 3]) * 3 + k % 3, s);
-        }
-      }
-      if (NEED_COL) {
-#pragma unroll
-        for (int k = 0; k < 9; ++k) {
-          float s = wave_sum(mine ? g_col[k] : 0.f);
-          if (lane == leader && s != 0.f) atomicAdd(a.g_col + ((size_t)b * a.N + vid[k / 3]) * 3 + k % 3, s);
-        }
-      }
+      const float s = any ? wave_total(sh_acc[k]) : 0.f;
+      if (lane == k) out[k] = s;
     }
   }
 }
@@ -335,7 +378,7 @@ __global__ void __launch_bounds__(256) k_face_jac(const float* __restrict__ vpos
 template <int ALBEDO, bool SHADED, bool M2D, bool HAVE_BARY>
 static void launch_pixel_grad_t(const BackwardArgs& a, hipStream_t s) {
   dim3 grid(a.tiles_x * a.tiles_y, a.B * a.C);
-  hipLaunchKernelGGL((k_pixel_grad<ALBEDO, SHADED, M2D, HAVE_BARY>), grid, dim3(256), 0, s, a);
+  hipLaunchKernelGGL((k_pixel_grad<ALBEDO, SHADED, M2D, HAVE_BARY>), grid, dim3(64), 0, s, a);
 }
 
 template <int ALBEDO, bool SHADED>

@@ -60,6 +60,8 @@ struct Handle {
   DevBuf<float> fnormal;
   DevBuf<float4> proj;
   DevBuf<unsigned long long> keys;
+  DevBuf<unsigned int> counters;
+  DevBuf<int2> large;
   DevBuf<float> g_vn;
   DevBuf<float> sh_partial;
   Profiler prof;
@@ -68,7 +70,7 @@ struct Handle {
   ~Handle() {
     d_faces.release(); d_csr_start.release(); d_csr_count.release(); d_csr_list.release();
     d_uv.release(); d_uv_table.release(); cams.release(); fnormal.release(); proj.release();
-    keys.release(); g_vn.release(); sh_partial.release();
+    keys.release(); counters.release(); large.release(); g_vn.release(); sh_partial.release();
   }
 };
 
@@ -212,6 +214,8 @@ int gvv_forward(void* handle, int32_t B, int32_t texH, int32_t texW, const float
   GVV_CHECK(h->fnormal.reserve((size_t)B * h->F * 3));
   GVV_CHECK(h->proj.reserve((size_t)B * h->C * h->N));
   GVV_CHECK(h->keys.reserve((size_t)pix));
+  GVV_CHECK(h->counters.reserve(2));
+  GVV_CHECK(h->large.reserve((size_t)B * h->C * h->F));
   if (h->keys.n != old_keys || old_keys == 0)
     GVV_CHECK(launch_fill_u64(h->keys.p, (int64_t)h->keys.n, kEmptyKey, s));
   if (h->normal_map && (h->uvW != texW || h->uvH != texH)) {
@@ -231,6 +235,7 @@ int gvv_forward(void* handle, int32_t B, int32_t texH, int32_t texW, const float
   p.faces = h->d_faces.p; p.csr_start = h->d_csr_start.p; p.csr_count = h->d_csr_count.p; p.csr_list = h->d_csr_list.p;
   p.uv_table = h->d_uv_table.p;
   p.cams = h->cams.p; p.fnormal = h->fnormal.p; p.proj = h->proj.p;
+  p.counters = h->counters.p; p.large = h->large.p;
   p.vnormal_out = vnormal;
   p.normal_map_out = normal_map;
   ResolveArgs& r = p.r;

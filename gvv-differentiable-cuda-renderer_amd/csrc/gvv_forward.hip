@@ -62,35 +62,142 @@ __global__ void __launch_bounds__(256) k_vertices(const float* __restrict__ vpos
   proj[i] = make_float4(p.x, p.y, p.z, 0.f);
 }
 
-// One thread per (b, c, face): walk the triangle's clamped bounding box (u outer, v inner as
-// the reference) and keep the smallest (depth key, face id) per pixel.
+// Conservative screen-space pre-test.  A pixel passes the exact ray/triangle test only if its
+// centre lies inside the projected triangle up to the float error of that test (~1e-4 px at
+// the supported scales), so every pixel farther than kEdgeMargin px outside an edge can be
+// skipped without changing any result.  The pre-test is disabled (every bounding-box pixel
+// takes the exact test) when a vertex is near or behind the camera plane, when coordinates
+// are huge, or when the projected triangle is degenerate.
+constexpr float kEdgeMargin = 0.5f;
+constexpr int64_t kLargeBox = 256;   // bounding boxes above this many pixels go to the workgroup queue
+constexpr int kLargeWorkgroups = 1024;
+
+struct EdgeTest {
+  float a[3], b[3], c[3];
+  bool on;
+  __device__ __forceinline__ bool pass(float x, float y) const {
+    if (!on) return true;
+    return (a[0] * x + b[0] * y + c[0] >= 0.f) && (a[1] * x + b[1] * y + c[1] >= 0.f) &&
+           (a[2] * x + b[2] * y + c[2] >= 0.f);
+  }
+};
+
+__device__ __forceinline__ EdgeTest edge_test(float4 p0, float4 p1, float4 p2) {
+  EdgeTest e;
+  e.on = false;
+  const float px[3] = {p0.x, p1.x, p2.x}, py[3] = {p0.y, p1.y, p2.y}, pz[3] = {p0.z, p1.z, p2.z};
+  for (int i = 0; i < 3; ++i)
+    if (!(pz[i] >= 1.0f) || !(fabsf(px[i]) <= 1e6f) || !(fabsf(py[i]) <= 1e6f)) return e;
+  const float area2 = (px[1] - px[0]) * (py[2] - py[0]) - (py[1] - py[0]) * (px[2] - px[0]);
+  if (!(fabsf(area2) >= 1e-3f)) return e;
+  const float s = area2 > 0.f ? 1.f : -1.f;
+  for (int i = 0; i < 3; ++i) {
+    const int j = (i + 1) % 3;
+    const float ex = px[j] - px[i], ey = py[j] - py[i];
+    const float len = sqrtf(ex * ex + ey * ey);
+    if (!(len > 0.f)) return e;
+    // signed distance of (x, y) to edge i, positive inside: s * cross(e, p - p_i) / |e|
+    e.a[i] = -s * ey / len;
+    e.b[i] = s * ex / len;
+    e.c[i] = -(e.a[i] * px[i] + e.b[i] * py[i]) + kEdgeMargin;
+  }
+  e.on = true;
+  return e;
+}
+
+struct RasterFace {
+  int f, i0, i1, i2;
+  float4 p0, p1, p2;
+  Box bx;
+};
+
+__device__ __forceinline__ RasterFace load_face(const int32_t* __restrict__ faces, const float4* __restrict__ pc,
+                                                int f, int W, int H) {
+  RasterFace r;
+  r.f = f;
+  r.i0 = faces[3 * f]; r.i1 = faces[3 * f + 1]; r.i2 = faces[3 * f + 2];
+  r.p0 = pc[r.i0]; r.p1 = pc[r.i1]; r.p2 = pc[r.i2];
+  r.bx = face_box(v3(r.p0.x, r.p0.y, r.p0.z), v3(r.p1.x, r.p1.y, r.p1.z), v3(r.p2.x, r.p2.y, r.p2.z), W, H);
+  return r;
+}
+
+// Exact test of pixel (u, v) against face r (reference uv2barycentric + inside test + depth key,
+// CUDABasedRasterization.cu:237-250), then the 64-bit minimum into the visibility buffer.
+__device__ __forceinline__ void raster_pixel(const RasterFace& r, const TriSetup& t, const Camera& cam, V3 o_s,
+                                             int u, int v, unsigned long long* kc, int W) {
+  V3 d = pixel_ray(cam, u + 0.5f, v + 0.5f);
+  V3 abc = ray_barycentric(t, o_s, d);
+  if (!inside(abc)) return;
+  uint64_t k = pack_key(depth_key(abc, r.p0.z, r.p1.z, r.p2.z), r.f);
+  atomicMin(kc + (size_t)v * W + u, (unsigned long long)k);
+}
+
+// One thread per (b, c, face) with a small bounding box: walk it (u outer, v inner as the
+// reference).  Faces with larger boxes are appended to `large` for k_raster_large.
 __global__ void __launch_bounds__(256) k_raster(const float* __restrict__ vpos, const int32_t* __restrict__ faces,
                                                 const float4* __restrict__ proj, const Camera* __restrict__ cams,
-                                                unsigned long long* __restrict__ keys, int B, int C, int N, int F,
-                                                int W, int H) {
+                                                unsigned long long* __restrict__ keys, unsigned int* __restrict__ large_count,
+                                                int2* __restrict__ large, int B, int C, int N, int F, int W, int H) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (int64_t)B * C * F) return;
-  int f = (int)(i % F);
-  int bc = (int)(i / F);
-  int b = bc / C;
-  int i0 = faces[3 * f], i1 = faces[3 * f + 1], i2 = faces[3 * f + 2];
-  const float4* pc = proj + (size_t)bc * N;
-  float4 p0 = pc[i0], p1 = pc[i1], p2 = pc[i2];
-  Box bx = face_box(v3(p0.x, p0.y, p0.z), v3(p1.x, p1.y, p1.z), v3(p2.x, p2.y, p2.z), W, H);
-  if (bx.x0 > bx.x1 || bx.y0 > bx.y1) return;
+  const int f = (int)(i % F);
+  const int bc = (int)(i / F);
+  const RasterFace r = load_face(faces, proj + (size_t)bc * N, f, W, H);
+  if (r.bx.x0 > r.bx.x1 || r.bx.y0 > r.bx.y1) return;
+  const int64_t area = (int64_t)(r.bx.x1 - r.bx.x0 + 1) * (r.bx.y1 - r.bx.y0 + 1);
+  if (area > kLargeBox) {
+    unsigned int slot = atomicAdd(large_count, 1u);
+    large[slot] = make_int2(bc, f);
+    return;
+  }
+  const int b = bc / C;
   const float* vp = vpos + (size_t)b * N * 3;
-  TriSetup t = tri_setup(ld3(vp + 3 * i0), ld3(vp + 3 * i1), ld3(vp + 3 * i2));
+  const TriSetup t = tri_setup(ld3(vp + 3 * r.i0), ld3(vp + 3 * r.i1), ld3(vp + 3 * r.i2));
   const Camera& cam = cams[bc];
-  V3 o_s = cam_origin(cam) / 1000.f;
+  const V3 o_s = cam_origin(cam) / 1000.f;
+  const EdgeTest et = edge_test(r.p0, r.p1, r.p2);
   unsigned long long* kc = keys + (size_t)bc * W * H;
-  for (int u = bx.x0; u <= bx.x1; ++u)
-    for (int v = bx.y0; v <= bx.y1; ++v) {
-      V3 d = pixel_ray(cam, u + 0.5f, v + 0.5f);
-      V3 abc = ray_barycentric(t, o_s, d);
-      if (!inside(abc)) continue;
-      uint64_t k = pack_key(depth_key(abc, p0.z, p1.z, p2.z), f);
-      atomicMin(kc + (size_t)v * W + u, (unsigned long long)k);
+  for (int u = r.bx.x0; u <= r.bx.x1; ++u)
+    for (int v = r.bx.y0; v <= r.bx.y1; ++v)
+      if (et.pass(u + 0.5f, v + 0.5f)) raster_pixel(r, t, cam, o_s, u, v, kc, W);
+}
+
+// Persistent workgroups drain the large-face queue: each takes one face at a time from an
+// atomic cursor and spreads its bounding box over the 256 threads.  Every workgroup leaves
+// when the cursor passes the queue length written by k_raster.
+__global__ void __launch_bounds__(256) k_raster_large(const float* __restrict__ vpos, const int32_t* __restrict__ faces,
+                                                      const float4* __restrict__ proj, const Camera* __restrict__ cams,
+                                                      unsigned long long* __restrict__ keys,
+                                                      unsigned int* __restrict__ counters, const int2* __restrict__ large,
+                                                      int C, int N, int W, int H) {
+  __shared__ unsigned int s_item;
+  const unsigned int n_large = counters[0];
+  for (;;) {
+    if (threadIdx.x == 0) s_item = atomicAdd(&counters[1], 1u);
+    __syncthreads();
+    const unsigned int item = s_item;
+    __syncthreads();
+    if (item >= n_large) break;
+    const int bc = large[item].x, f = large[item].y;
+    const int b = bc / C;
+    const RasterFace r = load_face(faces, proj + (size_t)bc * N, f, W, H);
+    const float* vp = vpos + (size_t)b * N * 3;
+    const TriSetup t = tri_setup(ld3(vp + 3 * r.i0), ld3(vp + 3 * r.i1), ld3(vp + 3 * r.i2));
+    const Camera& cam = cams[bc];
+    const V3 o_s = cam_origin(cam) / 1000.f;
+    const EdgeTest et = edge_test(r.p0, r.p1, r.p2);
+    unsigned long long* kc = keys + (size_t)bc * W * H;
+    const int bw = r.bx.x1 - r.bx.x0 + 1;
+    const int64_t area = (int64_t)bw * (r.bx.y1 - r.bx.y0 + 1);
+    for (int64_t p = threadIdx.x; p < area; p += blockDim.x) {
+      const int u = r.bx.x0 + (int)(p % bw), v = r.bx.y0 + (int)(p / bw);
+      if (et.pass(u + 0.5f, v + 0.5f)) raster_pixel(r, t, cam, o_s, u, v, kc, W);
     }
+  }
+}
+
+__global__ void k_zero_u32(unsigned int* p, int n) {
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0u;
 }
 
 template <int ALBEDO, bool SHADE>
@@ -222,9 +329,15 @@ hipError_t launch_forward(const ForwardPlan& p, hipStream_t s) {
     hipLaunchKernelGGL(k_normal_map, dim3(blocks_for((int64_t)B * T)), dim3(256), 0, s, p.uv_table, p.faces,
                        p.vnormal_out, p.normal_map_out, B, C, N, T);
   } else {
-    ProfScope ps(p.prof, "k_raster", s);
-    hipLaunchKernelGGL(k_raster, dim3(blocks_for((int64_t)B * C * F)), dim3(256), 0, s, p.vpos, p.faces, p.proj,
-                       p.cams, p.r.keys, B, C, N, F, p.r.W, p.r.H);
+    {
+      ProfScope ps(p.prof, "k_raster", s);
+      hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, p.counters, 2);
+      hipLaunchKernelGGL(k_raster, dim3(blocks_for((int64_t)B * C * F)), dim3(256), 0, s, p.vpos, p.faces, p.proj,
+                         p.cams, p.r.keys, p.counters, p.large, B, C, N, F, p.r.W, p.r.H);
+    }
+    ProfScope ps(p.prof, "k_raster_large", s);
+    hipLaunchKernelGGL(k_raster_large, dim3(kLargeWorkgroups), dim3(256), 0, s, p.vpos, p.faces, p.proj, p.cams,
+                       p.r.keys, p.counters, p.large, C, N, p.r.W, p.r.H);
   }
   // In normal-map mode the visibility buffer is empty, so this writes the background.
   ProfScope ps(p.prof, "k_resolve", s);

@@ -102,6 +102,8 @@ struct ForwardPlan {
   float4* proj;                      // [B][C][N]
   float* vnormal_out;                // [B][C][N][3]
   float* normal_map_out;             // [B][texH][texW][3]
+  unsigned int* counters;            // [2]: large-face queue length, drain cursor
+  int2* large;                       // [B*C*F] large-face queue of (b*C + c, face)
   ResolveArgs r;
   Profiler* prof;
 };

@@ -305,3 +305,41 @@ def random_soup(seed, n_faces=40, n_cameras=2, width=48, height=40, albedo_mode=
                  extr=np.tile(E[None], (batch, 1, 1)), intr=np.tile(K[None], (batch, 1, 1)),
                  target=target, width=width, height=height, albedo_mode=albedo_mode,
                  shading_mode=shading_mode, meta=dict(name=f"soup{seed}"))
+
+
+def stress_soup(seed, n_faces=80, width=64, height=48, albedo_mode="vertexColor"):
+    """Hard cases for visibility: triangles crossing the camera plane or behind the camera,
+    triangles covering most of the image, edge-on and sub-pixel triangles, coincident faces."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    E = look_at((0.0, 0.0, -3.0))
+    K = pinhole(width, height, 1.0)
+    tris = []
+    for i in range(n_faces):
+        kind = i % 5
+        if kind == 0:      # crossing / behind the camera plane (camera at z = -3 m)
+            c = np.array([rng.uniform(-0.5, 0.5), rng.uniform(-0.5, 0.5), rng.uniform(-3.5, -2.0)])
+            tris.append(c + rng.normal(0, 0.8, size=(3, 3)))
+        elif kind == 1:    # large, close
+            c = np.array([0.0, 0.0, rng.uniform(-2.0, 0.5)])
+            tris.append(c + rng.normal(0, 1.5, size=(3, 3)) * np.array([1, 1, 0.1]))
+        elif kind == 2:    # edge-on: all three vertices nearly in a plane through the camera ray
+            c = rng.uniform(-0.5, 0.5, size=3)
+            dz = rng.normal(0, 0.4)
+            tris.append(np.array([c, c + np.array([1e-4, 0, dz]), c + np.array([-1e-4, 1e-4, -dz])]))
+        elif kind == 3:    # sub-pixel
+            c = rng.uniform(-0.6, 0.6, size=3)
+            tris.append(c + rng.normal(0, 2e-3, size=(3, 3)))
+        else:              # ordinary
+            c = rng.uniform(-0.6, 0.6, size=3)
+            tris.append(c + rng.normal(0, 0.2, size=(3, 3)))
+    v = np.concatenate(tris, 0)
+    f = np.arange(v.shape[0], dtype=np.int32).reshape(-1, 3)
+    f = np.concatenate([f, f[:4]], 0)                      # coincident duplicates
+    N, F = v.shape[0], f.shape[0]
+    sh = sh_coefficients(1, rng)
+    return Scene(faces=f, uv=rng.uniform(0, 1, size=(F, 3, 2)).astype(np.float32),
+                 vpos=(v * MM).astype(np.float32)[None], vcol=rng.uniform(0, 1, size=(1, N, 3)).astype(np.float32),
+                 tex=rng.uniform(0, 1, size=(1, 16, 16, 3)).astype(np.float32), sh=sh[None],
+                 extr=E.reshape(1, 1, 12), intr=K.reshape(1, 1, 9),
+                 target=rng.uniform(0, 1, size=(1, 1, height, width, 3)).astype(np.float32),
+                 width=width, height=height, albedo_mode=albedo_mode, meta=dict(name=f"stress{seed}"))

@@ -105,3 +105,10 @@ def test_normal_map(device, tex_hw):
     e = rel_err(nmap.cpu().numpy(), ref["normal_map"])
     print(f"    normal map rel err {e:.3e}", flush=True)
     assert e <= RENDER_TOL
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2, 3])
+def test_stress_visibility(device, seed):
+    """Camera-plane crossings, near-full-screen, edge-on, sub-pixel and duplicate faces."""
+    s = scenes.stress_soup(seed)
+    _run(s, device, "vertexColor" if seed % 2 == 0 else "textured", "shaded", m2d=seed == 3, check_grads=seed < 2)
