@@ -48,73 +48,75 @@ __device__ __forceinline__ double dot(D3 a, D3 b) { return a.x * b.x + a.y * b.y
 __device__ __forceinline__ D3 cross(D3 a, D3 b) { return d3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
 
 // d(a,b,c)/d(v0,v1,v2) for a fixed ray (reference dJBCDVerpos, RendererUtil.h:670-861),
-// contracted with the row vector gb (gradient w.r.t. the three barycentrics).  Exploits that
-// dN/d(v_vtx . e_k) = e_k x A_vtx with A_0 = v1-v2, A_1 = v2-v0, A_2 = v0-v1, and that the
-// derivatives of the edges and of the vertices are unit vectors.  Adds the 9-vector gradient
-// w.r.t. (v0, v1, v2) to `out`.
+// contracted with the row vector gb (gradient w.r.t. the three barycentrics); adds the
+// 9-vector gradient w.r.t. (v0, v1, v2) to out[OFF .. OFF+8].
 //
-// The early exit is the reference's binary32 test.  The Jacobian itself is evaluated in FP64:
-// it scales like 1/(N.d) and several of its terms cancel near grazing incidence, so binary32
-// loses ~eps/cos relative accuracy there (measured 2.8e-4 max-norm on a 5k-face sphere, where
-// FP64 on binary32 inputs matches the FP64 oracle to ~1e-7).  It is ~150 flops per covered pixel.
-__device__ __forceinline__ void bary_position_grad(V3 of, V3 df, V3 v0f, V3 v1f, V3 v2f, V3 gbf, float out[9]) {
+// Regrouped form of the reference's per-variable loop.  With e1 = v1-v0, e2 = v2-v0,
+// N = e1 x e2, q0 = P - v0 (P the plane hit of the ray), q1 = q0 - e1, q2 = q0 - e2, the edges
+// E0 = v2-v1 and E1 = v0-v2, C_r = E_r x q_{r+1}, and w_r = gb_r - gb_2 (the third row is minus
+// the sum of the first two), the contribution of vertex v with A_0 = e1-e2, A_1 = e2, A_2 = -e1
+// (dN/d(v . e_k) = e_k x A_v) is
+//   [A_v x (w0 C0 + w1 C1) + Q_v + dt_v (w0 N.(E0 x d) + w1 N.(E1 x d)) - D_v] / |N|^2
+//   - 2 (A_v x N) (w0 N.C0 + w1 N.C1) / |N|^4
+// with dt_v = (delta_v0 N - A_v x q0) / (N.d) the derivative of the ray parameter,
+// Q_0 = w1 q2 x N, Q_1 = -w0 q1 x N, Q_2 = (w0 q1 - w1 q2) x N from the edge derivatives,
+// and D_1 = w0 N x E0, D_2 = w1 N x E1 (D_0 = 0) from the vertex derivatives.
+//
+// Precision: the early exit is the reference's binary32 test.  The plane hit q0 and N.d are
+// ill-conditioned near grazing incidence (relative error ~eps/cos in binary32, which showed
+// as 2.8e-4 max-norm position-gradient error on a 5k-face sphere), so those two are formed
+// in FP64 from exact vertex differences; everything after them is well conditioned and stays
+// binary32 (1.3e-7 max-norm against the FP64 oracle on the same sphere).
+template <int OFF>
+__device__ __forceinline__ void bary_position_grad(V3 of, V3 d, V3 v0f, V3 v1f, V3 v2f, V3 gb, float* out) {
+  const V3 e1 = v1f - v0f, e2 = v2f - v0f;
+  const V3 N = cross(e1, e2);
   {
-    V3 Nf = cross(v1f - v0f, v2f - v0f);
-    float denf = dot(Nf, Nf);
-    if (fabsf(dot(normalize(df), normalize(Nf))) < 0.001f || fabsf(denf * denf) < 0.001f) return;
+    const float denf = dot(N, N);
+    if (fabsf(dot(normalize(d), normalize(N))) < 0.001f || fabsf(denf * denf) < 0.001f) return;
   }
-  const D3 o = d3(of), d = d3(df), v0 = d3(v0f), v1 = d3(v1f), v2 = d3(v2f);
-  const D3 N = cross(v1 - v0, v2 - v0);
-  const double den = dot(N, N);
-  const double nd = dot(N, d);
-  const D3 w = v0 - o;
-  const double wN = dot(w, N);
-  const double t = wN / nd;                 // ray parameter of the plane hit: (v0 - o).N / N.d
-  const D3 P = o + t * d;
-  const D3 E[2] = {v2 - v1, v0 - v2};
-  const D3 vp[2] = {P - v1, P - v2};
-  const D3 Cr[2] = {cross(E[0], vp[0]), cross(E[1], vp[1])};
-  const double NC[2] = {dot(N, Cr[0]), dot(N, Cr[1])};
-  const D3 vpxN[2] = {cross(vp[0], N), cross(vp[1], N)};
-  const D3 NxE[2] = {cross(N, E[0]), cross(N, E[1])};
-  const double NEd[2] = {dot(N, cross(E[0], d)), dot(N, cross(E[1], d))};
-  // combine the two independent rows: J[2] = -J[0] - J[1]
-  const double w0 = (double)gbf.x - (double)gbf.z, w1 = (double)gbf.y - (double)gbf.z;
-  const double inv_den = 1.0 / den, inv_den2 = inv_den * inv_den, inv_nd = 1.0 / nd;
-  const double c_dt = -wN * inv_nd * inv_nd;
-  const D3 A[3] = {v1 - v2, v2 - v0, v0 - v1};
-  // sign of dE_r / dV_vtx: E_0 = v2 - v1, E_1 = v0 - v2
-  const double sE[2][3] = {{0.0, -1.0, 1.0}, {1.0, 0.0, -1.0}};
+  V3 q0;
+  float nd;
+  {
+    const D3 v0 = d3(v0f), dd = d3(d);
+    const D3 Nd = cross(d3(v1f) - v0, d3(v2f) - v0);
+    const D3 w = v0 - d3(of);
+    const double ndd = dot(Nd, dd);
+    const D3 q = (dot(w, Nd) / ndd) * dd - w;
+    q0 = v3((float)q.x, (float)q.y, (float)q.z);
+    nd = (float)ndd;
+  }
+  const float den = dot(N, N);
+  const V3 q1 = q0 - e1, q2 = q0 - e2;
+  const V3 E0 = e2 - e1, E1 = -e2;
+  const float w0 = gb.x - gb.z, w1 = gb.y - gb.z;
+  const V3 C0 = cross(E0, q1), C1 = cross(E1, q2);
+  const V3 Cw = w0 * C0 + w1 * C1;
+  const float NCw = w0 * dot(N, C0) + w1 * dot(N, C1);
+  const float NEdw = w0 * dot(N, cross(E0, d)) + w1 * dot(N, cross(E1, d));
+  const float inv_den = 1.f / den, inv_nd = 1.f / nd;
+  const float c2 = -2.f * NCw * inv_den * inv_den;
 #pragma unroll
-  for (int vtx = 0; vtx < 3; ++vtx) {
-    const D3 Axd = cross(A[vtx], d), Axw = cross(A[vtx], w), AxN = cross(A[vtx], N);
-    const D3 AxC0 = cross(A[vtx], Cr[0]), AxC1 = cross(A[vtx], Cr[1]);
-    const double axd[3] = {Axd.x, Axd.y, Axd.z}, axw[3] = {Axw.x, Axw.y, Axw.z}, axn[3] = {AxN.x, AxN.y, AxN.z};
-    const double axc[2][3] = {{AxC0.x, AxC0.y, AxC0.z}, {AxC1.x, AxC1.y, AxC1.z}};
-    const double nk[3] = {N.x, N.y, N.z};
-    const double vpn[2][3] = {{vpxN[0].x, vpxN[0].y, vpxN[0].z}, {vpxN[1].x, vpxN[1].y, vpxN[1].z}};
-    const double nxe[2][3] = {{NxE[0].x, NxE[0].y, NxE[0].z}, {NxE[1].x, NxE[1].y, NxE[1].z}};
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      const double dt = c_dt * axd[k] + inv_nd * ((vtx == 0 ? nk[k] : 0.0) + axw[k]);
-      const double NdN2 = 2.0 * axn[k];
-      double J[2];
-#pragma unroll
-      for (int r = 0; r < 2; ++r) {
-        const double dtmp = axc[r][k] + sE[r][vtx] * vpn[r][k] + dt * NEd[r] - (vtx == r + 1 ? nxe[r][k] : 0.0);
-        J[r] = dtmp * inv_den - NC[r] * inv_den2 * NdN2;
-      }
-      out[3 * vtx + k] += (float)(w0 * J[0] + w1 * J[1]);
-    }
+  for (int v = 0; v < 3; ++v) {
+    const V3 A = v == 0 ? e1 - e2 : (v == 1 ? e2 : -e1);
+    V3 dt = cross(A, q0) * (-inv_nd);
+    if (v == 0) dt = dt + N * inv_nd;
+    V3 Q = v == 0 ? cross(w1 * q2, N) : (v == 1 ? cross(-w0 * q1, N) : cross(w0 * q1 - w1 * q2, N));
+    if (v == 1) Q = Q - w0 * cross(N, E0);
+    if (v == 2) Q = Q - w1 * cross(N, E1);
+    const V3 g = inv_den * (cross(A, Cw) + Q + NEdw * dt) + c2 * cross(A, N);
+    out[OFF + 3 * v + 0] += g.x;
+    out[OFF + 3 * v + 1] += g.y;
+    out[OFF + 3 * v + 2] += g.z;
   }
 }
 
 // Per-pixel gradient terms of one covered pixel (reference Grad.cu:205-555).  Adds the SH
 // terms to sh_acc and the texture terms to g_tex directly; returns the per-face terms
 // (position, vertex-normal gradient and vertex-colour rows for the three corners).
-template <int ALBEDO, bool SHADED, bool M2D, bool HAVE_BARY>
+template <int ALBEDO, bool SHADED, bool M2D, bool HAVE_BARY, int G_POS, int G_VN, int G_COL>
 __device__ __forceinline__ void pixel_terms(const BackwardArgs& a, int bc, int b, int px, int py, size_t idx, int idf,
-                                            float g_pos[9], float g_vn[9], float g_col[9], float sh_acc[27]) {
+                                            float* t, float sh_acc[27]) {
   const Camera& cam = a.cams[bc];
   const V3 o = cam_origin(cam);
   const V3 d = pixel_ray(cam, px + 0.5f, py + 0.5f);
@@ -150,7 +152,7 @@ __device__ __forceinline__ void pixel_terms(const BackwardArgs& a, int bc, int b
     alb = c0 * abc.x + c1 * abc.y + c2 * abc.z;
     const V3 ga = v3(g.x * L.x, g.y * L.y, g.z * L.z);
 #pragma unroll
-    for (int r = 0; r < 3; ++r) { g_col[3 * r] = ga.x * w3[r]; g_col[3 * r + 1] = ga.y * w3[r]; g_col[3 * r + 2] = ga.z * w3[r]; }
+    for (int r = 0; r < 3; ++r) { t[G_COL + 3 * r] = ga.x * w3[r]; t[G_COL + 3 * r + 1] = ga.y * w3[r]; t[G_COL + 3 * r + 2] = ga.z * w3[r]; }
   } else if (ALBEDO == kTextured) {
     // the backward's albedo is the reference's four-texel blend (Grad.cu:282-312), not the
     // forward's nearest texel
@@ -196,9 +198,9 @@ __device__ __forceinline__ void pixel_terms(const BackwardArgs& a, int bc, int b
     const V3 gu = v3((gn[0] * unl2 - un.x * gnu) * inv_unl3, (gn[1] * unl2 - un.y * gnu) * inv_unl3,
                      (gn[2] * unl2 - un.z * gnu) * inv_unl3);
 #pragma unroll
-    for (int r = 0; r < 3; ++r) { g_vn[3 * r] = gu.x * w3[r]; g_vn[3 * r + 1] = gu.y * w3[r]; g_vn[3 * r + 2] = gu.z * w3[r]; }
+    for (int r = 0; r < 3; ++r) { t[G_VN + 3 * r] = gu.x * w3[r]; t[G_VN + 3 * r + 1] = gu.y * w3[r]; t[G_VN + 3 * r + 2] = gu.z * w3[r]; }
     const V3 gb = v3(dot(gu, n0), dot(gu, n1), dot(gu, n2));
-    bary_position_grad(o, d, p0, p1, p2, gb, g_pos);
+    bary_position_grad<G_POS>(o, d, p0, p1, p2, gb, t);
   }
 
   if (M2D) {
@@ -250,21 +252,30 @@ __device__ __forceinline__ void pixel_terms(const BackwardArgs& a, int bc, int b
         }
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
-          g_pos[3 * r] += gx[0] * w3[r]; g_pos[3 * r + 1] += gx[1] * w3[r]; g_pos[3 * r + 2] += gx[2] * w3[r];
+          t[G_POS + 3 * r] += gx[0] * w3[r]; t[G_POS + 3 * r + 1] += gx[1] * w3[r]; t[G_POS + 3 * r + 2] += gx[2] * w3[r];
         }
       }
     }
   }
 }
 
-// One wavefront per 16x16 pixel tile of one (b, c) image, in four 8x8 rounds.  Per-face terms
-// are summed over the lanes that share a face id (segmented DPP reduction) and one lane issues
-// the atomics for that face's three vertices; SH terms accumulate in registers over the whole
-// tile and are reduced once into the tile's partial slot.
+// One wavefront per 16x16 pixel tile of one (b, c) image, in four 8x8 rounds.
+// Per-face terms: every distinct face of the tile gets a slot in LDS (found by a ballot over
+// the slot table); lanes add their terms into their face's slot with LDS atomics, and the
+// slots are flushed with lane-parallel global atomics (one per face, vertex and component).
+// SH terms accumulate in registers over the tile and are reduced once into the tile's
+// partial slots.
+constexpr int kFaceSlots = 96;
+
 template <int ALBEDO, bool SHADED, bool M2D, bool HAVE_BARY>
-__global__ void __launch_bounds__(64) k_pixel_grad(BackwardArgs a) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_pixel_grad(BackwardArgs a) {
   constexpr bool NEED_POS = SHADED || M2D;
   constexpr bool NEED_COL = ALBEDO == kVertexColor;
+  constexpr int G_POS = 0, G_VN = NEED_POS ? 9 : 0, G_COL = G_VN + (SHADED ? 9 : 0);
+  constexpr int NT = G_COL + (NEED_COL ? 9 : 0);         // per-face terms
+  constexpr int NT1 = NT > 0 ? NT : 1;
+  __shared__ float s_acc[kFaceSlots][NT1];
+  __shared__ int s_face[kFaceSlots];
   const int lane = threadIdx.x;
   const int tile = blockIdx.x, bc = blockIdx.y;
   const int b = bc / a.C;
@@ -273,6 +284,22 @@ __global__ void __launch_bounds__(64) k_pixel_grad(BackwardArgs a) {
 #pragma unroll
   for (int k = 0; k < 27; ++k) sh_acc[k] = 0.f;
   bool any = false;
+  int nslots = 0;
+
+  auto flush = [&]() {
+    __syncthreads();
+    for (int i = lane; i < nslots * NT; i += 64) {
+      const int slot = i / NT, k = i % NT;
+      const float v = s_acc[slot][k];
+      if (v == 0.f) continue;
+      const int grp = k / 9, kk = k % 9;
+      const int vtx = a.faces[3 * s_face[slot] + kk / 3];
+      float* base = grp * 9 == G_POS && NEED_POS ? a.g_pos : (grp * 9 == G_VN && SHADED ? a.g_vn : a.g_col);
+      atomicAdd(base + ((size_t)b * a.N + vtx) * 3 + kk % 3, v);
+    }
+    __syncthreads();
+    nslots = 0;
+  };
 
   for (int round = 0; round < 4; ++round) {
     const int px = tx * 16 + (round & 1) * 8 + (lane & 7);
@@ -284,71 +311,69 @@ __global__ void __launch_bounds__(64) k_pixel_grad(BackwardArgs a) {
     uint64_t pending = __ballot(idf >= 0);
     if (!pending) continue;
     any = true;
-    float g_pos[9], g_vn[9], g_col[9];
+    float t[NT1 + 9];     // +9: unused groups of pixel_terms write past NT1 only in dead code
 #pragma unroll
-    for (int k = 0; k < 9; ++k) { g_pos[k] = 0.f; g_vn[k] = 0.f; g_col[k] = 0.f; }
-    if (idf >= 0) pixel_terms<ALBEDO, SHADED, M2D, HAVE_BARY>(a, bc, b, px, py, idx, idf, g_pos, g_vn, g_col, sh_acc);
-
-    if (NEED_POS || NEED_COL) {
-      while (pending) {
-        const int leader = __ffsll((unsigned long long)pending) - 1;
-        const int f = __builtin_amdgcn_readlane(idf, leader);
-        const bool mine = idf == f;
-        pending &= ~__ballot(mine);
-        const int v0 = a.faces[3 * f], v1 = a.faces[3 * f + 1], v2 = a.faces[3 * f + 2];
-        const int vid[3] = {v0, v1, v2};
-        if (NEED_POS) {
-#pragma unroll
-          for (int k = 0; k < 9; ++k) {
-            const float s = wave_total(mine ? g_pos[k] : 0.f);
-            if (lane == leader && s != 0.f) atomicAdd(a.g_pos + ((size_t)b * a.N + vid[k / 3]) * 3 + k % 3, s);
-          }
-        }
-        if (SHADED) {
-#pragma unroll
-          for (int k = 0; k < 9; ++k) {
-            const float s = wave_total(mine ? g_vn[k] : 0.f);
-            if (lane == leader && s != 0.f) atomicAdd(a.g_vn + ((size_t)b * a.N + vid[k / 3]) * 3 + k % 3, s);
-          }
-        }
-        if (NEED_COL) {
-#pragma unroll
-          for (int k = 0; k < 9; ++k) {
-            const float s = wave_total(mine ? g_col[k] : 0.f);
-            if (lane == leader && s != 0.f) atomicAdd(a.g_col + ((size_t)b * a.N + vid[k / 3]) * 3 + k % 3, s);
-          }
-        }
+    for (int k = 0; k < NT1 + 9; ++k) t[k] = 0.f;
+    if (idf >= 0)
+      pixel_terms<ALBEDO, SHADED, M2D, HAVE_BARY, G_POS, G_VN, G_COL>(a, bc, b, px, py, idx, idf, t, sh_acc);
+    if (NT == 0) continue;
+    // assign each distinct face of this round a slot (reusing slots of earlier rounds); a
+    // round adds at most 64 faces, so flushing first whenever fewer remain keeps every slot
+    // of this round valid until its terms are added
+    if (nslots > kFaceSlots - 64) flush();
+    int myslot = -1;
+    while (pending) {
+      const int leader = __ffsll((unsigned long long)pending) - 1;
+      const int f = __builtin_amdgcn_readlane(idf, leader);
+      const bool mine = idf == f;
+      pending &= ~__ballot(mine);
+      int slot = -1;
+      for (int base = 0; base < nslots && slot < 0; base += 64) {
+        const int j = base + lane;
+        const uint64_t hit = __ballot(j < nslots && s_face[j] == f);
+        if (hit) slot = base + __ffsll((unsigned long long)hit) - 1;
       }
+      if (slot < 0) {
+        slot = nslots++;
+        if (lane == 0) s_face[slot] = f;
+        for (int k = lane; k < NT; k += 64) s_acc[slot][k] = 0.f;
+      }
+      if (mine) myslot = slot;
+    }
+    __syncthreads();
+    if (myslot >= 0) {
+#pragma unroll
+      for (int k = 0; k < NT; ++k)
+        if (t[k] != 0.f) atomicAdd(&s_acc[myslot][k], t[k]);
     }
   }
+  if (NT > 0 && nslots > 0) flush();
 
   if (SHADED) {
-    float* out = a.sh_partial + ((size_t)bc * gridDim.x + tile) * 27;
+    const int tiles = gridDim.x;
 #pragma unroll
     for (int k = 0; k < 27; ++k) {
       const float s = any ? wave_total(sh_acc[k]) : 0.f;
-      if (lane == k) out[k] = s;
+      if (lane == k) a.sh_partial[((size_t)bc * 27 + k) * tiles + tile] = s;
     }
   }
 }
 
-// Fixed-order reduction of the SH tile partials: one workgroup per (b, c).
+// Fixed-order reduction of the SH tile partials ([b*C + c][27][tiles]): one workgroup per
+// (b, c, coefficient), so the result does not depend on scheduling.
 __global__ void __launch_bounds__(256) k_sh_reduce(const float* __restrict__ partial, float* __restrict__ g_sh, int tiles) {
   __shared__ float red[256];
-  const int bc = blockIdx.x;
-  const float* p = partial + (size_t)bc * tiles * 27;
-  for (int k = 0; k < 27; ++k) {
-    float s = 0.f;
-    for (int t = threadIdx.x; t < tiles; t += 256) s += p[(size_t)t * 27 + k];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-      if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) g_sh[(size_t)bc * 27 + k] = red[0];
+  const int bc = blockIdx.x, k = blockIdx.y;
+  const float* p = partial + ((size_t)bc * 27 + k) * tiles;
+  float s = 0.f;
+  for (int t = threadIdx.x; t < tiles; t += 256) s += p[t];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
     __syncthreads();
   }
+  if (threadIdx.x == 0) g_sh[(size_t)bc * 27 + k] = red[0];
 }
 
 // Normal -> position chain regrouped by face (see the file header).
@@ -422,7 +447,7 @@ hipError_t launch_backward(const BackwardPlan& p, hipStream_t s) {
   if (p.shaded) {
     {
       ProfScope ps(p.prof, "k_sh_reduce", s);
-      hipLaunchKernelGGL(k_sh_reduce, dim3(a.B * a.C), dim3(256), 0, s, a.sh_partial, a.g_sh, a.tiles_x * a.tiles_y);
+      hipLaunchKernelGGL(k_sh_reduce, dim3(a.B * a.C, 27), dim3(256), 0, s, a.sh_partial, a.g_sh, a.tiles_x * a.tiles_y);
     }
     ProfScope ps(p.prof, "k_face_jac", s);
     hipLaunchKernelGGL(k_face_jac, dim3(blocks_for((int64_t)a.B * a.F)), dim3(256), 0, s, a.vpos, a.faces, a.g_vn,
