@@ -11,6 +11,7 @@ and intrinsics are the row-major 3x3 K.
 """
 from __future__ import annotations
 
+import os
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -343,3 +344,28 @@ def stress_soup(seed, n_faces=80, width=64, height=48, albedo_mode="vertexColor"
                  extr=E.reshape(1, 1, 12), intr=K.reshape(1, 1, 9),
                  target=rng.uniform(0, 1, size=(1, 1, height, width, 3)).astype(np.float32),
                  width=width, height=height, albedo_mode=albedo_mode, meta=dict(name=f"stress{seed}"))
+
+
+# SH preset of the reference's demo scripts (python/data/test_SH_tensor.py:4): per channel
+# 0.7 ambient and -0.5 on the x band.
+REFERENCE_SH_PRESET = np.array([0.7, 0, 0, -0.5, 0, 0, 0, 0, 0] * 3, np.float32)
+
+
+def from_fixture(mesh, calib, width, height, batch=1, albedo_mode="vertexColor", shading_mode="shaded",
+                 texture=None, sh=None, seed=0):
+    """Scene from an OBJ mesh (gvv_amd.io.read_obj) and a calibration (gvv_amd.io.read_calibration),
+    with the reference demo's SH preset unless `sh` is given."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    C = calib.C
+    tex = texture if texture is not None else mesh.texture
+    if tex is None:
+        tex = smooth_texture(rng, 64, 64)
+    sh = REFERENCE_SH_PRESET if sh is None else np.asarray(sh, np.float32)
+    return Scene(faces=mesh.faces, uv=mesh.texcoords, vpos=np.tile(mesh.vertices[None], (batch, 1, 1)),
+                 vcol=np.tile(mesh.colors[None], (batch, 1, 1)),
+                 tex=np.tile(tex.astype(np.float32)[None], (batch, 1, 1, 1)),
+                 sh=np.tile(sh.reshape(1, 1, 27), (batch, C, 1)).astype(np.float32),
+                 extr=np.tile(calib.extrinsics[None], (batch, 1, 1)), intr=np.tile(calib.intrinsics[None], (batch, 1, 1)),
+                 target=rng.uniform(0, 1, size=(batch, C, height, width, 3)).astype(np.float32),
+                 width=width, height=height, albedo_mode=albedo_mode, shading_mode=shading_mode,
+                 meta=dict(name=os.path.basename(mesh.mtl_path or "mesh")))

@@ -429,3 +429,130 @@ def test_bary_jacobian_finite_difference():
             tm[var // 3, var % 3] -= 1e-3
             fd = (bary(tp) - bary(tm)) / 2e-3
             assert np.allclose(J[:, var], fd, rtol=1e-5, atol=1e-9)
+
+
+# ----------------------------------------------------------------------------------------------
+# further pins: normals, interpolation, normal map, textured-backward quirks, grazing exit
+# ----------------------------------------------------------------------------------------------
+def test_vertex_normals_are_incident_face_sums():
+    s = scenes.config0(width=32, height=32, level=2, n_cameras=1)
+    out = O.Oracle(**s.renderer_kwargs()).forward(*s.inputs())
+    v = s.vpos[0].astype(np.float64)
+    fn = np.cross(v[s.faces[:, 1]] - v[s.faces[:, 0]], v[s.faces[:, 2]] - v[s.faces[:, 0]])
+    vn = np.zeros_like(v)
+    for k in range(3):
+        np.add.at(vn, s.faces[:, k], fn)
+    got = out["vnormal"][0, 0].astype(np.float64)
+    assert np.allclose(got, vn, rtol=1e-5, atol=1e-3 * np.abs(vn).max())
+    assert ((got * v).sum(1) > 0).all()              # outward on the convex sphere
+
+
+def test_vertex_color_interpolation_identity_colours():
+    """Red/green/blue corners under unit lighting render exactly (a, b, 1-a-b)."""
+    s = scenes.single_triangle()                     # sh = DC 1 per channel, colours = identity
+    out = O.Oracle(**s.renderer_kwargs()).forward(*s.inputs())
+    m = out["face"][0, 0] >= 0
+    a = out["bary"][0, 0][..., 0][m].astype(np.float64)
+    b = out["bary"][0, 0][..., 1][m].astype(np.float64)
+    assert np.allclose(out["render"][0, 0][m], np.stack([a, b, 1 - a - b], 1), atol=1e-12)
+
+
+def test_normal_map_single_triangle():
+    """Texels inside the triangle's UV footprint get (n+1)/2 of its unit normal, the rest 0.5."""
+    s = scenes.single_triangle(albedo_mode="normal")
+    s.tex = np.zeros((1, 40, 32, 3), np.float32)
+    kw = s.renderer_kwargs()
+    out = O.Oracle(**kw, compute_normal_map=True).forward(*s.inputs())
+    nm = out["normal_map"][0]
+    assert (out["face"] == -1).all()                 # rasterisation is skipped in this mode
+    n = np.array([0.0, 0.0, 1.0])                    # face normal of the z=0 triangle
+    uv = s.uv[0].astype(np.float64) * np.array([32, 40]) * np.array([1, -1]) + np.array([0, 40])
+    T = np.array([[uv[0, 0], uv[1, 0], uv[2, 0]], [uv[0, 1], uv[1, 1], uv[2, 1]], [1, 1, 1]])
+    inside = outside = 0
+    for y in range(40):
+        for x in range(32):
+            w = np.linalg.solve(T, np.array([x + 0.5, y + 0.5, 1.0]))
+            if w.min() > 1e-3:
+                assert np.allclose(nm[y, x], (n + 1) / 2, atol=1e-12)
+                inside += 1
+            elif w.min() < -1e-3:
+                assert np.allclose(nm[y, x], 0.5, atol=1e-12)
+                outside += 1
+    assert inside > 100 and outside > 100
+
+
+def test_textured_backward_albedo_is_reflected_blend():
+    """Grad.cu:311-312 blends the four texels with swapped weights: on a ramp c = col index the
+    backward albedo is LU_index + 1 - s where U0 = LU_index + 0.5 + s (a standard bilinear blend
+    would give LU_index + s).  Read through the DC SH gradient of single pixels."""
+    s = scenes.single_triangle(albedo_mode="textured")
+    # reverse the winding so pixels are not flipped; normal (0,0,-1) faces the camera
+    s.faces = s.faces[:, ::-1].copy()
+    s.uv = s.uv[:, ::-1].copy()
+    th, tw = 8, 8
+    ramp = np.zeros((1, th, tw, 3), np.float32)
+    ramp[0, :, :, 0] = np.arange(tw)[None, :]
+    s.tex = ramp
+    orc = O.Oracle(**s.renderer_kwargs())
+    out = orc.forward(*s.inputs())
+    ys, xs = np.nonzero(out["face"][0, 0] >= 0)
+    checked = 0
+    for y, x in list(zip(ys, xs))[::37]:
+        g = np.zeros((1, 1, s.height, s.width, 3), np.float32)
+        g[0, 0, y, x, 0] = 1.0
+        gr = orc.backward(g, np.zeros_like(g), s.target, s.vpos, s.vcol, s.tex, s.sh, out["vnormal"], out["bary"],
+                          out["face"], s.extr, s.intr)
+        a, b = out["bary"][0, 0, y, x]
+        c = np.float32(1.0) - a - b
+        uv = s.uv[0]
+        u = (np.float32(uv[0, 0]) * a + np.float32(uv[1, 0]) * b) + np.float32(uv[2, 0]) * c
+        U = min(max(np.float32(u * np.float32(tw)), np.float32(0)), np.float32(tw - 1))
+        lu = int(np.trunc(U - np.float32(0.5)))
+        sfrac = float(U) - (lu + 0.5)
+        if lu + 1 >= tw:
+            continue
+        expect = lu + 1 - sfrac                          # x SH basis Y_0 = 1
+        assert abs(gr["sh"][0, 0, 0] - expect) < 1e-9, (y, x, gr["sh"][0, 0, 0], expect)
+        checked += 1
+    assert checked > 5
+
+
+def test_texture_gradient_skipped_for_flipped_pixels():
+    """Grad.cu:345: pixels whose interpolated normal faces away receive no texture gradient."""
+    s = scenes.single_triangle(albedo_mode="textured")       # winding gives +z: every pixel flipped
+    g = np.ones((1, 1, s.height, s.width, 3), np.float32)
+    orc = O.Oracle(**s.renderer_kwargs())
+    out = orc.forward(*s.inputs())
+    gr = orc.backward(g, np.zeros_like(g), s.target, s.vpos, s.vcol, s.tex, s.sh, out["vnormal"], out["bary"],
+                      out["face"], s.extr, s.intr)
+    assert (out["face"] >= 0).sum() > 100 and np.all(gr["tex"] == 0)
+    s.faces = s.faces[:, ::-1].copy()                        # reversed winding: nothing flipped
+    orc = O.Oracle(**s.renderer_kwargs())
+    out = orc.forward(*s.inputs())
+    gr = orc.backward(g, np.zeros_like(g), s.target, s.vpos, s.vcol, s.tex, s.sh, out["vnormal"], out["bary"],
+                      out["face"], s.extr, s.intr)
+    assert np.abs(gr["tex"]).sum() > 0
+
+
+def test_bary_jacobian_grazing_exit():
+    """|cos(ray, N)| < 0.001 returns a zero Jacobian (RendererUtil.h:682)."""
+    tri = np.array([[0, 0, 0], [100, 0, 0], [0, 100, 0]], np.float32)
+    o = np.array([30, 30, -3000], np.float32)
+    for cz, zero in ((5e-4, True), (5e-2, False)):
+        d = np.array([1.0, 0.0, cz], np.float64)
+        d /= np.linalg.norm(d)
+        J = O.bary_jacobian(o, d.astype(np.float32), tri.reshape(-1))
+        assert (np.abs(J).max() == 0.0) == zero
+
+
+def test_zero_gradients_in_normal_and_lighting_modes():
+    s = scenes.random_soup(2, n_faces=40, width=24, height=24)
+    for mode in ("normal", "lighting"):
+        kw = s.renderer_kwargs()
+        kw["albedo_mode"] = mode
+        orc = O.Oracle(**kw)
+        out = orc.forward(*s.inputs())
+        g = np.ones(out["render"].shape, np.float32)
+        gr = orc.backward(g, g, s.target, s.vpos, s.vcol, s.tex, s.sh, out["vnormal"], out["bary"], out["face"],
+                          s.extr, s.intr)
+        assert all(np.all(v == 0) for v in gr.values())
