@@ -84,6 +84,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-views", type=int, default=2)
     ap.add_argument("--kernels-json", default=None, help="also write per-kernel timings here")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
+                    help="after timing, run one plain device-to-device copy of MB MiB (reference kernel for "
+                         "scaling rocprof FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -143,6 +146,14 @@ def main():
         roofline = dict(bound="hbm", kernel=dom, achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                         frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, algorithmic_bytes=dom_bytes,
                         avg_ms=round(dom_ms, 4))
+
+    if args.calibrate_copy > 0:
+        n = args.calibrate_copy * (1 << 20) // 4
+        src = torch.ones(n, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     cpu = None
     if not args.no_cpu_baseline:
