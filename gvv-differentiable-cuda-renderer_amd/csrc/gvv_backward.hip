@@ -3,12 +3,12 @@
 // Replaces renderBuffersGradGPU (reference cpp/src/Renderer/CUDABasedRasterizationGrad.cu:624-635)
 // and its per-pixel Jacobian chains (Grad.cu:114-617, RendererUtil.h:241-914).
 //
-//   k_pixel_grad   one wavefront per 16x16 pixel tile (four 8x8 rounds).  Each covered pixel
-//                  forms its albedo, SH, barycentric-position, model-to-data and normal
-//                  terms.  Per-face terms are summed across the lanes that share a face id
-//                  (segmented DPP reduction) before a single lane issues the global atomics
-//                  for the face's three vertices; SH terms accumulate in registers over the
-//                  tile and are written once per tile into a partial buffer.
+//   k_pixel_grad   one wavefront per TILE x TILE pixel tile (TILE 8/16/32, in 8x8 rounds).
+//                  Each covered pixel forms its albedo, SH, barycentric-position,
+//                  model-to-data and normal terms.  Per-face terms are summed in LDS slots
+//                  (one per distinct face of the tile) before lane-parallel global atomics
+//                  flush them to the face's three vertices; SH terms accumulate in registers
+//                  over the tile and are reduced once per tile (DPP) into a partial buffer.
 //   k_sh_reduce    fixed-order reduction of the SH partials (deterministic).
 //   k_face_jac     the normal -> position chain.  Instead of walking every incident face of
 //                  every corner for every pixel (Grad.cu:559-615), the pixels accumulate
@@ -259,7 +259,8 @@ __device__ __forceinline__ void pixel_terms(const BackwardArgs& a, int bc, int b
   }
 }
 
-// One wavefront per 16x16 pixel tile of one (b, c) image, in four 8x8 rounds.
+// One wavefront per TILE x TILE pixel tile of one (b, c) image (TILE = 8, 16 or 32), in
+// (TILE/8)^2 rounds of 8x8 pixels.
 // Per-face terms: every distinct face of the tile gets a slot in LDS (found by a ballot over
 // the slot table); lanes add their terms into their face's slot with LDS atomics, and the
 // slots are flushed with lane-parallel global atomics (one per face, vertex and component).
@@ -267,8 +268,10 @@ __device__ __forceinline__ void pixel_terms(const BackwardArgs& a, int bc, int b
 // partial slots.
 constexpr int kFaceSlots = 96;
 
-template <int ALBEDO, bool SHADED, bool M2D, bool HAVE_BARY>
+template <int ALBEDO, bool SHADED, bool M2D, bool HAVE_BARY, int TILE>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_pixel_grad(BackwardArgs a) {
+  static_assert(TILE == 8 || TILE == 16 || TILE == 32, "tile edge must be 8, 16 or 32");
+  constexpr int RT = TILE / 8;                            // 8x8 rounds per tile edge
   constexpr bool NEED_POS = SHADED || M2D;
   constexpr bool NEED_COL = ALBEDO == kVertexColor;
   constexpr int G_POS = 0, G_VN = NEED_POS ? 9 : 0, G_COL = G_VN + (SHADED ? 9 : 0);
@@ -301,9 +304,9 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
     nslots = 0;
   };
 
-  for (int round = 0; round < 4; ++round) {
-    const int px = tx * 16 + (round & 1) * 8 + (lane & 7);
-    const int py = ty * 16 + (round >> 1) * 8 + (lane >> 3);
+  for (int round = 0; round < RT * RT; ++round) {
+    const int px = tx * TILE + (round % RT) * 8 + (lane & 7);
+    const int py = ty * TILE + (round / RT) * 8 + (lane >> 3);
     const bool valid = px < a.W && py < a.H;
     const size_t idx = ((size_t)bc * a.H + (valid ? py : 0)) * a.W + (valid ? px : 0);
     int idf = valid ? a.face[idx] : -1;
@@ -403,7 +406,9 @@ __global__ void __launch_bounds__(256) k_face_jac(const float* __restrict__ vpos
 template <int ALBEDO, bool SHADED, bool M2D, bool HAVE_BARY>
 static void launch_pixel_grad_t(const BackwardArgs& a, hipStream_t s) {
   dim3 grid(a.tiles_x * a.tiles_y, a.B * a.C);
-  hipLaunchKernelGGL((k_pixel_grad<ALBEDO, SHADED, M2D, HAVE_BARY>), grid, dim3(64), 0, s, a);
+  if (a.tile == 8) hipLaunchKernelGGL((k_pixel_grad<ALBEDO, SHADED, M2D, HAVE_BARY, 8>), grid, dim3(64), 0, s, a);
+  else if (a.tile == 32) hipLaunchKernelGGL((k_pixel_grad<ALBEDO, SHADED, M2D, HAVE_BARY, 32>), grid, dim3(64), 0, s, a);
+  else hipLaunchKernelGGL((k_pixel_grad<ALBEDO, SHADED, M2D, HAVE_BARY, 16>), grid, dim3(64), 0, s, a);
 }
 
 template <int ALBEDO, bool SHADED>

@@ -14,65 +14,28 @@
 #include <string>
 #include <vector>
 
-#include "gvv_kernels.hpp"
+#include "gvv_handle.hpp"
 #include "../../include/gvv_renderer.h"
+
+namespace {
+thread_local std::string g_err;
+}  // namespace
+
+namespace gvv {
+int set_last_error(const std::string& m) { g_err = m; return 1; }
+}  // namespace gvv
 
 namespace {
 
 using namespace gvv;
 
-thread_local std::string g_err;
-int fail(const std::string& m) { g_err = m; return 1; }
+int fail(const std::string& m) { return set_last_error(m); }
 
 #define GVV_CHECK(expr)                                                              \
   do {                                                                               \
     hipError_t _e = (expr);                                                          \
     if (_e != hipSuccess) return fail(std::string(#expr) + ": " + hipGetErrorString(_e)); \
   } while (0)
-
-template <class T>
-struct DevBuf {
-  T* p = nullptr;
-  size_t n = 0;
-  hipError_t reserve(size_t count) {
-    if (count <= n) return hipSuccess;
-    if (p) (void)hipFree(p);
-    p = nullptr; n = 0;
-    hipError_t e = hipMalloc((void**)&p, sizeof(T) * (count ? count : 1));
-    if (e == hipSuccess) n = count;
-    return e;
-  }
-  void release() { if (p) (void)hipFree(p); p = nullptr; n = 0; }
-};
-
-struct Handle {
-  int N = 0, F = 0, C = 0, W = 0, H = 0;
-  int albedo = 0, shading = 0, image_fs = 1, texture_fs = 1, normal_map = 0;
-  int device = 0;
-  std::vector<int32_t> faces;
-  std::vector<float> uv;
-  DevBuf<int32_t> d_faces, d_csr_start, d_csr_count, d_csr_list;
-  DevBuf<float> d_uv;
-  int uvW = -1, uvH = -1;
-  DevBuf<UvTexel> d_uv_table;
-  // scratch
-  DevBuf<Camera> cams;
-  DevBuf<float> fnormal;
-  DevBuf<float4> proj;
-  DevBuf<unsigned long long> keys;
-  DevBuf<unsigned int> counters;
-  DevBuf<int2> large;
-  DevBuf<float> g_vn;
-  DevBuf<float> sh_partial;
-  Profiler prof;
-  std::mutex mu;
-
-  ~Handle() {
-    d_faces.release(); d_csr_start.release(); d_csr_count.release(); d_csr_list.release();
-    d_uv.release(); d_uv_table.release(); cams.release(); fnormal.release(); proj.release();
-    keys.release(); counters.release(); large.release(); g_vn.release(); sh_partial.release();
-  }
-};
 
 // Vertex->face lists in increasing face order, each face once per vertex
 // (reference CUDABasedRasterization.cpp:125-154), built by counting sort in O(N + F).
@@ -132,9 +95,89 @@ int check_device(const Handle* h) {
   return 0;
 }
 
-inline int tiles(int n) { return (n + 15) / 16; }
+inline int tiles(int n, int tile) { return (n + tile - 1) / tile; }
 
 }  // namespace
+
+namespace gvv {
+
+int plan_forward(Handle* h, int B, int texH, int texW, const float* vpos, const float* vcol, const float* tex,
+                 const float* sh, const float* extr, const float* intr, float* bary, int32_t* face, float* render,
+                 float* vnormal, float* normal_map, hipStream_t s, ForwardPlan& p) {
+  if (B <= 0) return fail("batch size must be > 0");
+  if (texH <= 0 || texW <= 0) return fail("texture height/width must be > 0");
+  if (!vpos || !vcol || !tex || !sh || !extr || !intr || !bary || !face || !render || !vnormal)
+    return fail("null tensor pointer");
+  if (h->normal_map && !normal_map) return fail("compute_normal_map is set but normal_map is null");
+  const int64_t pix = (int64_t)B * h->C * h->W * h->H;
+  size_t old_keys = h->keys.n;
+  GVV_CHECK(h->cams.reserve((size_t)B * h->C));
+  GVV_CHECK(h->fnormal.reserve((size_t)B * h->F * 3));
+  GVV_CHECK(h->proj.reserve((size_t)B * h->C * h->N));
+  GVV_CHECK(h->keys.reserve((size_t)pix));
+  GVV_CHECK(h->counters.reserve(2));
+  GVV_CHECK(h->large.reserve((size_t)B * h->C * h->F));
+  if (h->keys.n != old_keys || old_keys == 0)
+    GVV_CHECK(launch_fill_u64(h->keys.p, (int64_t)h->keys.n, kEmptyKey, s));
+  if (h->normal_map && (h->uvW != texW || h->uvH != texH)) {
+    std::vector<UvTexel> table;
+    build_uv_table(*h, texW, texH, table);
+    GVV_CHECK(h->d_uv_table.reserve(table.size()));
+    GVV_CHECK(hipMemcpyAsync(h->d_uv_table.p, table.data(), sizeof(UvTexel) * table.size(), hipMemcpyHostToDevice, s));
+    GVV_CHECK(hipStreamSynchronize(s));
+    h->uvW = texW; h->uvH = texH;
+  }
+  p.B = B; p.C = h->C; p.N = h->N; p.F = h->F;
+  p.albedo = h->albedo;
+  p.shade = (h->shading == kShaded && h->albedo != kNormal) || h->albedo == kLighting;
+  p.normal_map = h->normal_map != 0;
+  p.vpos = vpos; p.extr = extr; p.intr = intr;
+  p.faces = h->d_faces.p; p.csr_start = h->d_csr_start.p; p.csr_count = h->d_csr_count.p; p.csr_list = h->d_csr_list.p;
+  p.uv_table = h->d_uv_table.p;
+  p.cams = h->cams.p; p.fnormal = h->fnormal.p; p.proj = h->proj.p;
+  p.counters = h->counters.p; p.large = h->large.p; p.large_box = h->large_box;
+  p.vnormal_out = vnormal;
+  p.normal_map_out = normal_map;
+  ResolveArgs& r = p.r;
+  r.B = B; r.C = h->C; r.N = h->N; r.W = h->W; r.H = h->H; r.texW = texW; r.texH = texH;
+  r.keys = h->keys.p; r.vpos = vpos; r.vcol = vcol; r.tex = tex; r.sh = sh; r.uv = h->d_uv.p;
+  r.faces = h->d_faces.p; r.cams = h->cams.p; r.vnormal = vnormal;
+  r.face = face; r.bary = bary; r.render = render;
+  p.prof = nullptr;
+  return 0;
+}
+
+int plan_backward(Handle* h, int B, int texH, int texW, const float* g_render, const float* g_target,
+                  const float* target, const float* vpos, const float* vcol, const float* tex, const float* sh,
+                  const float* vnormal, const float* bary, const int32_t* face, const float* extr,
+                  const float* intr, float* g_pos, float* g_col, float* g_tex, float* g_sh, BackwardPlan& p) {
+  if (B <= 0) return fail("batch size must be > 0");
+  if (texH <= 0 || texW <= 0) return fail("texture height/width must be > 0");
+  if (!g_pos || !g_col || !g_tex || !g_sh) return fail("null gradient output pointer");
+  if (h->albedo == kNormal || h->albedo == kLighting) return fail("normal and lighting modes have no gradients");
+  const int64_t nv = (int64_t)B * h->N * 3;
+  if (!g_render || !vpos || !vcol || !tex || !sh || !vnormal || !face || !extr || !intr)
+    return fail("null tensor pointer");
+  if (g_target && !target) return fail("target_buffer_grad given without target_image");
+  const int tx = tiles(h->W, h->tile), ty = tiles(h->H, h->tile);
+  GVV_CHECK(h->cams.reserve((size_t)B * h->C));
+  GVV_CHECK(h->g_vn.reserve((size_t)nv));
+  GVV_CHECK(h->sh_partial.reserve((size_t)B * h->C * tx * ty * 27));
+  p.albedo = h->albedo;
+  p.shaded = h->shading == kShaded;
+  p.extr = extr; p.intr = intr; p.cams = h->cams.p;
+  p.prof = nullptr;
+  BackwardArgs& a = p.a;
+  a.B = B; a.C = h->C; a.N = h->N; a.F = h->F; a.W = h->W; a.H = h->H; a.texW = texW; a.texH = texH;
+  a.image_fs = h->image_fs; a.tile = h->tile; a.tiles_x = tx; a.tiles_y = ty;
+  a.g_render = g_render; a.g_target = g_target; a.target = target;
+  a.vpos = vpos; a.vcol = vcol; a.tex = tex; a.sh = sh; a.vnormal = vnormal; a.bary = bary; a.face = face;
+  a.faces = h->d_faces.p; a.uv = h->d_uv.p; a.cams = h->cams.p;
+  a.g_pos = g_pos; a.g_col = g_col; a.g_tex = g_tex; a.g_vn = h->g_vn.p; a.sh_partial = h->sh_partial.p; a.g_sh = g_sh;
+  return 0;
+}
+
+}  // namespace gvv
 
 extern "C" {
 
@@ -202,47 +245,10 @@ int gvv_forward(void* handle, int32_t B, int32_t texH, int32_t texW, const float
   if (!h) return fail("null handle");
   std::lock_guard<std::mutex> lock(h->mu);
   if (check_device(h)) return 1;
-  if (B <= 0) return fail("batch size must be > 0");
-  if (texH <= 0 || texW <= 0) return fail("texture height/width must be > 0");
-  if (!vpos || !vcol || !tex || !sh || !extr || !intr || !bary || !face || !render || !vnormal)
-    return fail("null tensor pointer");
-  if (h->normal_map && !normal_map) return fail("compute_normal_map is set but normal_map is null");
   hipStream_t s = (hipStream_t)hip_stream;
-  const int64_t pix = (int64_t)B * h->C * h->W * h->H;
-  size_t old_keys = h->keys.n;
-  GVV_CHECK(h->cams.reserve((size_t)B * h->C));
-  GVV_CHECK(h->fnormal.reserve((size_t)B * h->F * 3));
-  GVV_CHECK(h->proj.reserve((size_t)B * h->C * h->N));
-  GVV_CHECK(h->keys.reserve((size_t)pix));
-  GVV_CHECK(h->counters.reserve(2));
-  GVV_CHECK(h->large.reserve((size_t)B * h->C * h->F));
-  if (h->keys.n != old_keys || old_keys == 0)
-    GVV_CHECK(launch_fill_u64(h->keys.p, (int64_t)h->keys.n, kEmptyKey, s));
-  if (h->normal_map && (h->uvW != texW || h->uvH != texH)) {
-    std::vector<UvTexel> table;
-    build_uv_table(*h, texW, texH, table);
-    GVV_CHECK(h->d_uv_table.reserve(table.size()));
-    GVV_CHECK(hipMemcpyAsync(h->d_uv_table.p, table.data(), sizeof(UvTexel) * table.size(), hipMemcpyHostToDevice, s));
-    GVV_CHECK(hipStreamSynchronize(s));
-    h->uvW = texW; h->uvH = texH;
-  }
   ForwardPlan p;
-  p.B = B; p.C = h->C; p.N = h->N; p.F = h->F;
-  p.albedo = h->albedo;
-  p.shade = (h->shading == kShaded && h->albedo != kNormal) || h->albedo == kLighting;
-  p.normal_map = h->normal_map != 0;
-  p.vpos = vpos; p.extr = extr; p.intr = intr;
-  p.faces = h->d_faces.p; p.csr_start = h->d_csr_start.p; p.csr_count = h->d_csr_count.p; p.csr_list = h->d_csr_list.p;
-  p.uv_table = h->d_uv_table.p;
-  p.cams = h->cams.p; p.fnormal = h->fnormal.p; p.proj = h->proj.p;
-  p.counters = h->counters.p; p.large = h->large.p;
-  p.vnormal_out = vnormal;
-  p.normal_map_out = normal_map;
-  ResolveArgs& r = p.r;
-  r.B = B; r.C = h->C; r.N = h->N; r.W = h->W; r.H = h->H; r.texW = texW; r.texH = texH;
-  r.keys = h->keys.p; r.vpos = vpos; r.vcol = vcol; r.tex = tex; r.sh = sh; r.uv = h->d_uv.p;
-  r.faces = h->d_faces.p; r.cams = h->cams.p; r.vnormal = vnormal;
-  r.face = face; r.bary = bary; r.render = render;
+  if (plan_forward(h, B, texH, texW, vpos, vcol, tex, sh, extr, intr, bary, face, render, vnormal, normal_map, s, p))
+    return 1;
   p.prof = h->prof.enabled ? &h->prof : nullptr;
   GVV_CHECK(launch_forward(p, s));
   return 0;
@@ -269,25 +275,11 @@ int gvv_backward(void* handle, int32_t B, int32_t texH, int32_t texW, const floa
     GVV_CHECK(launch_fill_f32(g_sh, (int64_t)B * h->C * 27, 0.f, s));
     return 0;
   }
-  if (!g_render || !vpos || !vcol || !tex || !sh || !vnormal || !face || !extr || !intr)
-    return fail("null tensor pointer");
-  if (g_target && !target) return fail("target_buffer_grad given without target_image");
-  const int tx = tiles(h->W), ty = tiles(h->H);
-  GVV_CHECK(h->cams.reserve((size_t)B * h->C));
-  GVV_CHECK(h->g_vn.reserve((size_t)nv));
-  GVV_CHECK(h->sh_partial.reserve((size_t)B * h->C * tx * ty * 27));
   BackwardPlan p;
-  p.albedo = h->albedo;
-  p.shaded = h->shading == kShaded;
-  p.extr = extr; p.intr = intr; p.cams = h->cams.p;
+  if (plan_backward(h, B, texH, texW, g_render, g_target, target, vpos, vcol, tex, sh, vnormal, bary, face, extr, intr,
+                    g_pos, g_col, g_tex, g_sh, p))
+    return 1;
   p.prof = h->prof.enabled ? &h->prof : nullptr;
-  BackwardArgs& a = p.a;
-  a.B = B; a.C = h->C; a.N = h->N; a.F = h->F; a.W = h->W; a.H = h->H; a.texW = texW; a.texH = texH;
-  a.image_fs = h->image_fs; a.tiles_x = tx; a.tiles_y = ty;
-  a.g_render = g_render; a.g_target = g_target; a.target = target;
-  a.vpos = vpos; a.vcol = vcol; a.tex = tex; a.sh = sh; a.vnormal = vnormal; a.bary = bary; a.face = face;
-  a.faces = h->d_faces.p; a.uv = h->d_uv.p; a.cams = h->cams.p;
-  a.g_pos = g_pos; a.g_col = g_col; a.g_tex = g_tex; a.g_vn = h->g_vn.p; a.sh_partial = h->sh_partial.p; a.g_sh = g_sh;
   GVV_CHECK(launch_backward(p, s));
   return 0;
 }

@@ -69,8 +69,7 @@ __global__ void __launch_bounds__(256) k_vertices(const float* __restrict__ vpos
 // takes the exact test) when a vertex is near or behind the camera plane, when coordinates
 // are huge, or when the projected triangle is degenerate.
 constexpr float kEdgeMargin = 0.5f;
-constexpr int64_t kLargeBox = 256;   // bounding boxes above this many pixels go to the workgroup queue
-constexpr int kLargeWorkgroups = 1024;
+constexpr int kLargeWorkgroups = 4096;
 
 struct EdgeTest {
   float a[3], b[3], c[3];
@@ -121,78 +120,170 @@ __device__ __forceinline__ RasterFace load_face(const int32_t* __restrict__ face
   return r;
 }
 
-// Exact test of pixel (u, v) against face r (reference uv2barycentric + inside test + depth key,
+// Per-face constants of the exact test, staged in LDS for the candidate pass.
+struct FaceStage {
+  TriSetup t;           // scaled vertices, N, v0.N, N.N
+  float z0, z1, z2;     // projected depths
+  int f, x0, y0, bw, bc;
+};
+
+// Candidate list of one wavefront: (face slot << 24 | pixel index inside the face's box).
+constexpr int kCandCap = 1536;
+
+// Exact test of every staged candidate (reference uv2barycentric + inside test + depth key,
 // CUDABasedRasterization.cu:237-250), then the 64-bit minimum into the visibility buffer.
-__device__ __forceinline__ void raster_pixel(const RasterFace& r, const TriSetup& t, const Camera& cam, V3 o_s,
-                                             int u, int v, unsigned long long* kc, int W) {
-  V3 d = pixel_ray(cam, u + 0.5f, v + 0.5f);
-  V3 abc = ray_barycentric(t, o_s, d);
-  if (!inside(abc)) return;
-  uint64_t k = pack_key(depth_key(abc, r.p0.z, r.p1.z, r.p2.z), r.f);
-  atomicMin(kc + (size_t)v * W + u, (unsigned long long)k);
-}
-
-// One thread per (b, c, face) with a small bounding box: walk it (u outer, v inner as the
-// reference).  Faces with larger boxes are appended to `large` for k_raster_large.
-__global__ void __launch_bounds__(256) k_raster(const float* __restrict__ vpos, const int32_t* __restrict__ faces,
-                                                const float4* __restrict__ proj, const Camera* __restrict__ cams,
-                                                unsigned long long* __restrict__ keys, unsigned int* __restrict__ large_count,
-                                                int2* __restrict__ large, int B, int C, int N, int F, int W, int H) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (int64_t)B * C * F) return;
-  const int f = (int)(i % F);
-  const int bc = (int)(i / F);
-  const RasterFace r = load_face(faces, proj + (size_t)bc * N, f, W, H);
-  if (r.bx.x0 > r.bx.x1 || r.bx.y0 > r.bx.y1) return;
-  const int64_t area = (int64_t)(r.bx.x1 - r.bx.x0 + 1) * (r.bx.y1 - r.bx.y0 + 1);
-  if (area > kLargeBox) {
-    unsigned int slot = atomicAdd(large_count, 1u);
-    large[slot] = make_int2(bc, f);
-    return;
+// Each lane takes candidates lane, lane+64, ...; all lanes run the expensive test together.
+__device__ __forceinline__ void flush_candidates(const FaceStage* st, const unsigned* cand, int count,
+                                                 const Camera* __restrict__ cams, bool one_view, int bc0,
+                                                 unsigned long long* __restrict__ keys, int W, int H) {
+  const int lane = threadIdx.x & 63;
+  for (int i = lane; i < count; i += 64) {
+    const unsigned c = cand[i];
+    const FaceStage& fs = st[c >> 24];
+    const int p = (int)(c & 0xffffffu);
+    const int dy = p / fs.bw;
+    const int u = fs.x0 + (p - dy * fs.bw), v = fs.y0 + dy;
+    const Camera& cam = cams[one_view ? bc0 : fs.bc];
+    const V3 d = pixel_ray(cam, u + 0.5f, v + 0.5f);
+    const V3 abc = ray_barycentric(fs.t, cam_origin(cam) / 1000.f, d);
+    if (!inside(abc)) continue;
+    const uint64_t k = pack_key(depth_key(abc, fs.z0, fs.z1, fs.z2), fs.f);
+    atomicMin(keys + ((size_t)fs.bc * H + v) * W + u, (unsigned long long)k);
   }
-  const int b = bc / C;
-  const float* vp = vpos + (size_t)b * N * 3;
-  const TriSetup t = tri_setup(ld3(vp + 3 * r.i0), ld3(vp + 3 * r.i1), ld3(vp + 3 * r.i2));
-  const Camera& cam = cams[bc];
-  const V3 o_s = cam_origin(cam) / 1000.f;
-  const EdgeTest et = edge_test(r.p0, r.p1, r.p2);
-  unsigned long long* kc = keys + (size_t)bc * W * H;
-  for (int u = r.bx.x0; u <= r.bx.x1; ++u)
-    for (int v = r.bx.y0; v <= r.bx.y1; ++v)
-      if (et.pass(u + 0.5f, v + 0.5f)) raster_pixel(r, t, cam, o_s, u, v, kc, W);
 }
 
-// Persistent workgroups drain the large-face queue: each takes one face at a time from an
-// atomic cursor and spreads its bounding box over the 256 threads.  Every workgroup leaves
-// when the cursor passes the queue length written by k_raster.
-__global__ void __launch_bounds__(256) k_raster_large(const float* __restrict__ vpos, const int32_t* __restrict__ faces,
-                                                      const float4* __restrict__ proj, const Camera* __restrict__ cams,
-                                                      unsigned long long* __restrict__ keys,
-                                                      unsigned int* __restrict__ counters, const int2* __restrict__ large,
-                                                      int C, int N, int W, int H) {
-  __shared__ unsigned int s_item;
+// Pre-test the box of staged face `slot` lane-parallel (64 pixels per round) and append the
+// passing pixels to the candidate list; flushes first whenever a round could overflow it.
+__device__ __forceinline__ void collect_face(const EdgeTest& et_in, int slot, int x0, int y0, int bw, int bh,
+                                             FaceStage* st, unsigned* cand, int& count, const Camera* __restrict__ cams,
+                                             bool one_view, int bc0, unsigned long long* __restrict__ keys, int W, int H) {
+  const int lane = threadIdx.x & 63;
+  const int area = bw * bh;
+  for (int base = 0; base < area; base += 64) {
+    if (count > kCandCap - 64) {
+      __syncthreads();
+      flush_candidates(st, cand, count, cams, one_view, bc0, keys, W, H);
+      __syncthreads();
+      count = 0;
+    }
+    const int p = base + lane;
+    bool pass = false;
+    if (p < area) {
+      const int dy = p / bw;
+      pass = et_in.pass(x0 + (p - dy * bw) + 0.5f, y0 + dy + 0.5f);
+    }
+    const uint64_t m = __ballot(pass);
+    if (pass) {
+      const int off = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+      cand[count + off] = ((unsigned)slot << 24) | (unsigned)p;
+    }
+    count += __popcll(m);
+  }
+}
+
+__device__ __forceinline__ EdgeTest readlane_edges(const EdgeTest& e, int j) {
+  EdgeTest r;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    r.a[i] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(e.a[i]), j));
+    r.b[i] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(e.b[i]), j));
+    r.c[i] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(e.c[i]), j));
+  }
+  r.on = __builtin_amdgcn_readlane((int)e.on, j) != 0;
+  return r;
+}
+
+// One wavefront per 64 consecutive (b, c, face) items.  Faces with bounding boxes up to
+// large_box pixels are staged and rasterised here (pre-test, compaction, exact test); larger
+// ones are appended to `large` for k_raster_large.
+__global__ void __launch_bounds__(64) k_raster(const float* __restrict__ vpos, const int32_t* __restrict__ faces,
+                                               const float4* __restrict__ proj, const Camera* __restrict__ cams,
+                                               unsigned long long* __restrict__ keys, unsigned int* __restrict__ large_count,
+                                               int2* __restrict__ large, int B, int C, int N, int F, int W, int H,
+                                               int large_box) {
+  __shared__ FaceStage st[64];
+  __shared__ unsigned cand[kCandCap];
+  const int lane = threadIdx.x;
+  const int64_t i = (int64_t)blockIdx.x * 64 + lane;
+  const bool in_range = i < (int64_t)B * C * F;
+  const int f = in_range ? (int)(i % F) : 0;
+  const int bc = in_range ? (int)(i / F) : 0;
+  bool small = false;
+  EdgeTest et;
+  et.on = false;
+  int bw = 0, bh = 0;
+  if (in_range) {
+    const RasterFace r = load_face(faces, proj + (size_t)bc * N, f, W, H);
+    if (r.bx.x0 <= r.bx.x1 && r.bx.y0 <= r.bx.y1) {
+      bw = r.bx.x1 - r.bx.x0 + 1;
+      bh = r.bx.y1 - r.bx.y0 + 1;
+      if ((int64_t)bw * bh > large_box) {
+        const unsigned int slot = atomicAdd(large_count, 1u);
+        large[slot] = make_int2(bc, f);
+      } else {
+        small = true;
+        const float* vp = vpos + (size_t)(bc / C) * N * 3;
+        FaceStage s;
+        s.t = tri_setup(ld3(vp + 3 * r.i0), ld3(vp + 3 * r.i1), ld3(vp + 3 * r.i2));
+        s.z0 = r.p0.z; s.z1 = r.p1.z; s.z2 = r.p2.z;
+        s.f = f; s.x0 = r.bx.x0; s.y0 = r.bx.y0; s.bw = bw; s.bc = bc;
+        st[lane] = s;
+        et = edge_test(r.p0, r.p1, r.p2);
+      }
+    }
+  }
+  uint64_t todo = __ballot(small);
+  if (!todo) return;
+  const int bc0 = __builtin_amdgcn_readfirstlane(bc);
+  const bool one_view = __ballot(bc != bc0 && in_range) == 0;
+  __syncthreads();
+  int count = 0;
+  while (todo) {
+    const int j = __ffsll((unsigned long long)todo) - 1;
+    todo &= todo - 1;
+    const EdgeTest ej = readlane_edges(et, j);
+    const int x0 = __builtin_amdgcn_readlane(st[j].x0, 0), y0 = __builtin_amdgcn_readlane(st[j].y0, 0);
+    const int jbw = __builtin_amdgcn_readlane(bw, j), jbh = __builtin_amdgcn_readlane(bh, j);
+    collect_face(ej, j, x0, y0, jbw, jbh, st, cand, count, cams, one_view, bc0, keys, W, H);
+  }
+  __syncthreads();
+  flush_candidates(st, cand, count, cams, one_view, bc0, keys, W, H);
+}
+
+// Persistent wavefronts drain the large-face queue: each takes one face at a time from an
+// atomic cursor and runs the same pre-test / compaction / exact-test passes over its box.
+// Every wavefront leaves when the cursor passes the queue length written by k_raster.
+__global__ void __launch_bounds__(64) k_raster_large(const float* __restrict__ vpos, const int32_t* __restrict__ faces,
+                                                     const float4* __restrict__ proj, const Camera* __restrict__ cams,
+                                                     unsigned long long* __restrict__ keys,
+                                                     unsigned int* __restrict__ counters, const int2* __restrict__ large,
+                                                     int C, int N, int W, int H) {
+  __shared__ FaceStage st[1];
+  __shared__ unsigned cand[kCandCap];
   const unsigned int n_large = counters[0];
   for (;;) {
-    if (threadIdx.x == 0) s_item = atomicAdd(&counters[1], 1u);
-    __syncthreads();
-    const unsigned int item = s_item;
-    __syncthreads();
+    unsigned int item = 0;
+    if (threadIdx.x == 0) item = atomicAdd(&counters[1], 1u);
+    item = __builtin_amdgcn_readfirstlane(item);
     if (item >= n_large) break;
     const int bc = large[item].x, f = large[item].y;
-    const int b = bc / C;
     const RasterFace r = load_face(faces, proj + (size_t)bc * N, f, W, H);
-    const float* vp = vpos + (size_t)b * N * 3;
-    const TriSetup t = tri_setup(ld3(vp + 3 * r.i0), ld3(vp + 3 * r.i1), ld3(vp + 3 * r.i2));
-    const Camera& cam = cams[bc];
-    const V3 o_s = cam_origin(cam) / 1000.f;
-    const EdgeTest et = edge_test(r.p0, r.p1, r.p2);
-    unsigned long long* kc = keys + (size_t)bc * W * H;
-    const int bw = r.bx.x1 - r.bx.x0 + 1;
-    const int64_t area = (int64_t)bw * (r.bx.y1 - r.bx.y0 + 1);
-    for (int64_t p = threadIdx.x; p < area; p += blockDim.x) {
-      const int u = r.bx.x0 + (int)(p % bw), v = r.bx.y0 + (int)(p / bw);
-      if (et.pass(u + 0.5f, v + 0.5f)) raster_pixel(r, t, cam, o_s, u, v, kc, W);
+    const float* vp = vpos + (size_t)(bc / C) * N * 3;
+    const int bw = r.bx.x1 - r.bx.x0 + 1, bh = r.bx.y1 - r.bx.y0 + 1;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      FaceStage s;
+      s.t = tri_setup(ld3(vp + 3 * r.i0), ld3(vp + 3 * r.i1), ld3(vp + 3 * r.i2));
+      s.z0 = r.p0.z; s.z1 = r.p1.z; s.z2 = r.p2.z;
+      s.f = f; s.x0 = r.bx.x0; s.y0 = r.bx.y0; s.bw = bw; s.bc = bc;
+      st[0] = s;
     }
+    const EdgeTest et = edge_test(r.p0, r.p1, r.p2);
+    __syncthreads();
+    int count = 0;
+    collect_face(et, 0, r.bx.x0, r.bx.y0, bw, bh, st, cand, count, cams, true, bc, keys, W, H);
+    __syncthreads();
+    flush_candidates(st, cand, count, cams, true, bc, keys, W, H);
   }
 }
 
@@ -332,11 +423,11 @@ hipError_t launch_forward(const ForwardPlan& p, hipStream_t s) {
     {
       ProfScope ps(p.prof, "k_raster", s);
       hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, p.counters, 2);
-      hipLaunchKernelGGL(k_raster, dim3(blocks_for((int64_t)B * C * F)), dim3(256), 0, s, p.vpos, p.faces, p.proj,
-                         p.cams, p.r.keys, p.counters, p.large, B, C, N, F, p.r.W, p.r.H);
+      hipLaunchKernelGGL(k_raster, dim3(blocks_for((int64_t)B * C * F, 64)), dim3(64), 0, s, p.vpos, p.faces, p.proj,
+                         p.cams, p.r.keys, p.counters, p.large, B, C, N, F, p.r.W, p.r.H, p.large_box);
     }
     ProfScope ps(p.prof, "k_raster_large", s);
-    hipLaunchKernelGGL(k_raster_large, dim3(kLargeWorkgroups), dim3(256), 0, s, p.vpos, p.faces, p.proj, p.cams,
+    hipLaunchKernelGGL(k_raster_large, dim3(kLargeWorkgroups), dim3(64), 0, s, p.vpos, p.faces, p.proj, p.cams,
                        p.r.keys, p.counters, p.large, C, N, p.r.W, p.r.H);
   }
   // In normal-map mode the visibility buffer is empty, so this writes the background.

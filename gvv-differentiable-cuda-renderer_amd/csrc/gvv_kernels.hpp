@@ -104,12 +104,14 @@ struct ForwardPlan {
   float* normal_map_out;             // [B][texH][texW][3]
   unsigned int* counters;            // [2]: large-face queue length, drain cursor
   int2* large;                       // [B*C*F] large-face queue of (b*C + c, face)
+  int large_box;                     // bounding boxes above this many pixels go to the queue
   ResolveArgs r;
   Profiler* prof;
 };
 
 struct BackwardArgs {
   int B, C, N, F, W, H, texW, texH, image_fs;
+  int tile;                          // pixel tile edge of k_pixel_grad: 8, 16 or 32
   int tiles_x, tiles_y;
   const float* g_render;             // [B][C][H][W][3]
   const float* g_target;             // [B][C][H][W][3] or null (no model-to-data term)

@@ -1,4 +1,5 @@
-"""ctypes binding of libgvv_amd.so (C ABI declared in include/gvv_renderer.h).
+"""ctypes binding of libgvv_amd.so (C ABI declared in include/gvv_renderer.h and
+include/gvv_proposal.h).
 
 The library is the only compute path: if it has not been built this module raises; there is
 no CPU fallback.
@@ -12,7 +13,9 @@ _PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.environ.get("GVV_AMD_LIB", os.path.join(_PKG, "libgvv_amd.so"))
 
 EXPORTS = ("gvv_last_error", "gvv_version", "gvv_create", "gvv_destroy", "gvv_forward", "gvv_backward",
-           "gvv_profile_enable", "gvv_profile_report")
+           "gvv_profile_enable", "gvv_profile_report",
+           "gvv_conv_out_size", "gvv_conv_forward", "gvv_conv_backward_data", "gvv_conv_workspace_floats",
+           "gvv_conv_backward_weight", "gvv_conv_prepare_weights", "gvv_pack_mosaic")
 
 _lib = None
 
@@ -45,6 +48,20 @@ def load():
     L.gvv_profile_enable.restype = ctypes.c_int
     L.gvv_profile_report.argtypes = [P, ctypes.c_char_p, i64]
     L.gvv_profile_report.restype = i64
+    L.gvv_conv_out_size.argtypes = [i32] * 4
+    L.gvv_conv_out_size.restype = ctypes.c_int
+    L.gvv_conv_forward.argtypes = [i32] * 8 + [P, P, P, i32, P, i32, P]
+    L.gvv_conv_forward.restype = ctypes.c_int
+    L.gvv_conv_backward_data.argtypes = [i32] * 8 + [P, P, P, P, P]
+    L.gvv_conv_backward_data.restype = ctypes.c_int
+    L.gvv_conv_workspace_floats.argtypes = [i32] * 8
+    L.gvv_conv_workspace_floats.restype = i64
+    L.gvv_conv_backward_weight.argtypes = [i32] * 8 + [P, P, P, P, P, i64, P]
+    L.gvv_conv_backward_weight.restype = ctypes.c_int
+    L.gvv_conv_prepare_weights.argtypes = [i32, i32, i32, P, P, P, P]
+    L.gvv_conv_prepare_weights.restype = ctypes.c_int
+    L.gvv_pack_mosaic.argtypes = [P, i32, i32, i32, i32, i32, i32, P, P]
+    L.gvv_pack_mosaic.restype = ctypes.c_int
     _lib = L
     return L
 

@@ -1,0 +1,224 @@
+"""Analysis-by-synthesis fitting through the differentiable rasterizer.
+
+The reference's own use of the renderer is a fit loop: render, compare with a target image,
+back-propagate and step an optimizer (python/test_gradients_VertexColor.py:93-138,
+test_gradients_SphericalHarmonics.py:95-130).  This module runs that loop at the scale of the
+workload configuration it is measured on:
+
+* a template mesh (capped cylinder, 175 x 200 grid + 2 poles = 35,002 vertices / 70,000 faces)
+  with an 80-dimensional linear shape basis of smooth seeded displacement fields,
+  orthonormalised and scaled to 0.02 m per unit coefficient;
+* per subject: 80 shape coefficients, 27 SH coefficients (shared by its views) and a 6-DoF
+  pose (axis-angle rotation about the template centre, translation in units of 0.1 m).  The
+  pose is applied to the vertices before rendering: the reference returns zero gradients
+  for extrinsics (python/CudaRenderer.py:215), so rigid motion has to live upstream;
+* vertexColor + shaded rendering of 4 views at 512 x 512, masked L2 photometric loss against
+  targets rendered from ground-truth parameters, Adam (lr 1e-2).
+
+Lengths are millimetres inside the renderer (the reference's thresholds assume mm).
+"""
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import scenes
+from .renderer import Rasterizer, RenderFunction
+
+MM = 1000.0
+
+
+@dataclass
+class FitConfig:
+    n_basis: int = 80
+    basis_scale_m: float = 0.02
+    cols: int = 175
+    rows: int = 200
+    views: int = 4
+    resolution: int = 512
+    batch: int = 16
+    iters: int = 200
+    lr: float = 1e-2
+    seed: int = 0
+
+
+def _smooth_fields(rng, faces, n, k, iters=60):
+    """k random 3-vector fields on the mesh vertices, smoothed by `iters` steps of uniform
+    Laplacian averaging over the face adjacency (smooth, low-frequency displacements)."""
+    import scipy.sparse as sp
+    rows = np.concatenate([faces[:, 0], faces[:, 1], faces[:, 2], faces[:, 1], faces[:, 2], faces[:, 0]])
+    cols = np.concatenate([faces[:, 1], faces[:, 2], faces[:, 0], faces[:, 0], faces[:, 1], faces[:, 2]])
+    A = sp.coo_matrix((np.ones(rows.size), (rows, cols)), shape=(n, n)).tocsr()
+    A.data[:] = 1.0
+    deg = np.asarray(A.sum(1)).ravel()
+    M = sp.diags(1.0 / np.maximum(deg, 1)) @ A
+    X = rng.normal(size=(n, 3 * k))
+    for _ in range(iters):
+        X = 0.5 * X + 0.5 * (M @ X)
+    return X.reshape(n, k, 3).transpose(1, 0, 2)          # (k, n, 3)
+
+
+def rodrigues(w: torch.Tensor) -> torch.Tensor:
+    """(B, 3) axis-angle -> (B, 3, 3) rotation matrices, smooth at zero."""
+    th2 = (w * w).sum(-1, keepdim=True)
+    th = torch.sqrt(th2 + 1e-24)
+    small = th2 < 1e-8
+    a = torch.where(small, 1.0 - th2 / 6.0, torch.sin(th) / th)
+    b = torch.where(small, 0.5 - th2 / 24.0, (1.0 - torch.cos(th)) / (th2 + 1e-24))
+    zero = torch.zeros_like(w[:, 0])
+    K = torch.stack([zero, -w[:, 2], w[:, 1], w[:, 2], zero, -w[:, 0], -w[:, 1], w[:, 0], zero], -1).view(-1, 3, 3)
+    eye = torch.eye(3, device=w.device, dtype=w.dtype).expand(w.shape[0], 3, 3)
+    return eye + a[..., None] * K + b[..., None] * (K @ K)
+
+
+@dataclass
+class Params:
+    alpha: torch.Tensor        # (B, K) shape coefficients
+    sh: torch.Tensor           # (B, 27)
+    pose: torch.Tensor         # (B, 6): axis-angle (rad), translation (0.1 m units)
+
+    def detach(self):
+        return Params(self.alpha.detach().clone(), self.sh.detach().clone(), self.pose.detach().clone())
+
+    def flat(self):
+        return torch.cat([self.alpha, self.sh, self.pose], 1)
+
+    @staticmethod
+    def from_flat(x, n_basis):
+        return Params(x[:, :n_basis], x[:, n_basis:n_basis + 27], x[:, n_basis + 27:n_basis + 33])
+
+
+class ShapeModel:
+    """Template, shape basis, colours and cameras of the fitting workload; renders parameters."""
+
+    def __init__(self, cfg: FitConfig = FitConfig(), device=None):
+        self.cfg = cfg
+        self.device = device or torch.device("cuda", torch.cuda.current_device())
+        r_geo, r_basis, r_col = np.random.Generator(np.random.PCG64(cfg.seed)).spawn(3)
+        v, f, uv = scenes.capped_cylinder(cfg.cols, cfg.rows, 1.0, 2.0, r_geo, 0.05)
+        self.faces = f
+        self.uv = uv
+        self.N, self.F = v.shape[0], f.shape[0]
+        B = _smooth_fields(r_basis, f, self.N, cfg.n_basis).reshape(cfg.n_basis, -1).T     # (3N, K)
+        Q, _ = np.linalg.qr(B)
+        Q = Q.T.reshape(cfg.n_basis, self.N, 3) * (cfg.basis_scale_m * MM * math.sqrt(self.N))
+        dev = self.device
+        self.template = torch.tensor(v * MM, dtype=torch.float32, device=dev)
+        self.centre = self.template.mean(0)
+        self.basis = torch.tensor(Q.reshape(cfg.n_basis, -1), dtype=torch.float32, device=dev)   # (K, 3N)
+        self.colors = torch.tensor(r_col.uniform(0.2, 0.9, size=(self.N, 3)), dtype=torch.float32, device=dev)
+        E, K = scenes.ring_cameras(cfg.views, 3.5, cfg.resolution, cfg.resolution)
+        self.extr = torch.tensor(E, device=dev)
+        self.intr = torch.tensor(K, device=dev)
+        self.rast = Rasterizer(f.reshape(-1), uv.reshape(-1), self.N, cfg.views, cfg.resolution, cfg.resolution,
+                               albedo_mode="vertexColor", shading_mode="shaded")
+        self._tex = torch.full((1, 1, 1, 3), 0.5, device=dev)
+        self._target_dummy = {}
+
+    # -- geometry ------------------------------------------------------------------------------
+    def vertices(self, p: Params) -> torch.Tensor:
+        B = p.alpha.shape[0]
+        X = self.template + (p.alpha @ self.basis).view(B, self.N, 3)
+        R = rodrigues(p.pose[:, :3])
+        t = p.pose[:, 3:6] * (0.1 * MM)
+        return torch.matmul(X - self.centre, R.transpose(1, 2).contiguous()) + self.centre + t[:, None, :]
+
+    def render(self, p: Params):
+        """Returns (render [B,C,H,W,3], face ids [B,C,H,W]) for parameters p (differentiable)."""
+        out = self.render_all(p)
+        return out[2], out[1]
+
+    def render_all(self, p: Params):
+        """All six renderer outputs for parameters p (differentiable)."""
+        B = p.alpha.shape[0]
+        C, R = self.cfg.views, self.cfg.resolution
+        vpos = self.vertices(p)
+        vcol = self.colors.expand(B, self.N, 3)
+        sh = p.sh[:, None, :].expand(B, C, 27)
+        tex = self._tex.expand(B, 1, 1, 3)
+        key = B
+        if key not in self._target_dummy:
+            self._target_dummy[key] = torch.zeros((B, C, R, R, 3), device=self.device)
+        return RenderFunction.apply(self.rast, vpos, vcol, tex, sh, self._target_dummy[key],
+                                    self.extr.expand(B, C, 12).reshape(B, C * 12),
+                                    self.intr.expand(B, C, 9).reshape(B, C * 9))
+
+
+def sample_params(rng: np.random.Generator, B: int, n_basis: int, device) -> Params:
+    """Ground-truth prior: shape ~ N(0, 0.5^2) per coefficient, SH DC ~ U(0.6, 1.0) with the
+    other coefficients ~ N(0, 0.1^2), rotation ~ N(0, 0.1^2) rad per axis, translation
+    ~ N(0, 0.3^2) x 0.1 m per axis."""
+    alpha = rng.normal(0.0, 0.5, size=(B, n_basis))
+    sh = rng.normal(0.0, 0.1, size=(B, 27))
+    sh[:, [0, 9, 18]] = rng.uniform(0.6, 1.0, size=(B, 3))
+    pose = np.concatenate([rng.normal(0.0, 0.1, size=(B, 3)), rng.normal(0.0, 0.3, size=(B, 3))], 1)
+    t = lambda a: torch.tensor(a, dtype=torch.float32, device=device)
+    return Params(t(alpha), t(sh), t(pose))
+
+
+def plain_init(B: int, n_basis: int, device) -> Params:
+    """The plain fit's start: zero shape and pose, neutral white DC lighting."""
+    sh = torch.zeros((B, 27), device=device)
+    sh[:, [0, 9, 18]] = 0.8
+    return Params(torch.zeros((B, n_basis), device=device), sh, torch.zeros((B, 6), device=device))
+
+
+def masked_l2(render, target, mask):
+    """Mean squared photometric error over the target's foreground pixels, per subject summed."""
+    d = (render - target) * mask
+    return (d * d).sum() / mask.sum().clamp_min(1.0)
+
+
+@dataclass
+class FitResult:
+    params: Params
+    losses: list = field(default_factory=list)
+    times: list = field(default_factory=list)          # wall-clock seconds after each iteration
+
+    @property
+    def final_loss(self):
+        return self.losses[-1]
+
+    def time_to(self, loss_level):
+        """Wall-clock seconds until the loss first reached loss_level (None if never)."""
+        for l, t in zip(self.losses, self.times):
+            if l <= loss_level:
+                return t
+        return None
+
+
+def fit(model: ShapeModel, target, mask, init: Params, iters=None, lr=None, stop_at=None, t0=None) -> FitResult:
+    """Adam over (shape, SH, pose).  `losses[i]` is the loss of the parameters before step i+1;
+    `times[i]` the wall-clock since t0 (default: entry) after evaluating it.  Stops early once
+    the loss is <= stop_at."""
+    cfg = model.cfg
+    iters = cfg.iters if iters is None else iters
+    lr = cfg.lr if lr is None else lr
+    x = init.flat().detach().clone().requires_grad_(True)
+    opt = torch.optim.Adam([x], lr=lr)
+    res = FitResult(params=None)
+    t0 = time.perf_counter() if t0 is None else t0
+    for _ in range(iters):
+        opt.zero_grad(set_to_none=True)
+        render, _ = model.render(Params.from_flat(x, cfg.n_basis))
+        loss = masked_l2(render, target, mask)
+        loss.backward()
+        lv = float(loss.detach())
+        res.losses.append(lv)
+        res.times.append(time.perf_counter() - t0)
+        if stop_at is not None and lv <= stop_at:
+            break
+        opt.step()
+    res.params = Params.from_flat(x.detach(), cfg.n_basis)
+    return res
+
+
+def make_targets(model: ShapeModel, gt: Params):
+    with torch.no_grad():
+        render, face = model.render(gt)
+    mask = (face >= 0).to(torch.float32).unsqueeze(-1)
+    return render.contiguous(), mask

@@ -58,16 +58,18 @@ class Rasterizer:
         if rc != 0:
             raise ValueError(L.gvv_last_error().decode())
         self._h = h
+        self._destroy = L.gvv_destroy      # kept: module globals may be gone at interpreter exit
         self.faces_np = faces.reshape(-1, 3)
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h:
+        destroy = getattr(self, "_destroy", None)
+        if h and destroy is not None:
             try:
                 torch.cuda.synchronize()
-            except Exception:
+            except Exception:      # interpreter shutdown: torch may already be torn down
                 pass
-            _lib.load().gvv_destroy(h)
+            destroy(h)
             self._h = None
 
     # -- shapes ------------------------------------------------------------------------------

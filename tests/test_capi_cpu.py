@@ -1,5 +1,6 @@
 """The C ABI without a GPU: libgvv_amd.so loads, exports every function include/gvv_renderer.h
-declares, and rejects invalid arguments on the host before any device call."""
+and include/gvv_proposal.h declare, and rejects invalid arguments on the host before any
+device call."""
 import ctypes
 import os
 import re
@@ -11,17 +12,18 @@ from gvv_amd import _lib
 from gvv_amd.renderer import Rasterizer
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADER = os.path.join(ROOT, "include", "gvv_renderer.h")
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("gvv_renderer.h", "gvv_proposal.h")]
 
 
 def declared_functions():
-    text = open(HEADER).read()
+    text = "".join(open(h).read() for h in HEADERS)
     return sorted(set(re.findall(r"^\s*(?:const\s+)?(?:int|void|char|int64_t)\s*\*?\s*(gvv_\w+)\s*\(", text, re.M)))
 
 
 def test_header_declares_the_abi():
     names = declared_functions()
-    for n in ("gvv_create", "gvv_destroy", "gvv_forward", "gvv_backward", "gvv_last_error"):
+    for n in ("gvv_create", "gvv_destroy", "gvv_forward", "gvv_backward", "gvv_last_error",
+              "gvv_conv_forward", "gvv_conv_backward_data", "gvv_conv_backward_weight"):
         assert n in names
 
 
@@ -70,3 +72,21 @@ def test_null_handle_calls_fail_cleanly():
 def test_python_rejects_mode_strings(albedo, shading):
     with pytest.raises(ValueError):
         Rasterizer([0, 1, 2], [0.0] * 6, 3, 1, 8, 8, albedo_mode=albedo, shading_mode=shading)
+
+
+def test_conv_rejects_invalid_geometry():
+    L = _lib.load()
+    P = ctypes.c_void_p(16)     # never dereferenced: validation fails first
+    assert L.gvv_conv_out_size(256, 3, 2, 1) == 128
+    assert L.gvv_conv_out_size(4, 1, 1, 0) == 4
+    rc = L.gvv_conv_forward(2, 8, 8, 3, 32, 3, 2, 1, P, P, P, 1, P, 0, None)
+    assert rc != 0 and "multiples of 8" in L.gvv_last_error().decode()
+    rc = L.gvv_conv_forward(2, 2, 2, 8, 32, 5, 1, 0, P, P, P, 1, P, 0, None)
+    assert rc != 0 and "kernel larger" in L.gvv_last_error().decode()
+    rc = L.gvv_conv_forward(2, 8, 8, 8, 32, 3, 2, 1, ctypes.c_void_p(18), P, P, 1, P, 0, None)
+    assert rc != 0 and "aligned" in L.gvv_last_error().decode()
+    need = L.gvv_conv_workspace_floats(2, 8, 8, 8, 32, 3, 2, 1)
+    assert need > 0
+    rc = L.gvv_conv_backward_weight(2, 8, 8, 8, 32, 3, 2, 1, P, P, P, P, P, need - 1, None)
+    assert rc != 0 and "workspace" in L.gvv_last_error().decode()
+    assert L.gvv_conv_workspace_floats(2, 8, 8, 7, 32, 3, 2, 1) == -1
