@@ -8,8 +8,19 @@ every input already resident in HBM; the upstream gradient is the sum-of-squares
 2(render - target) of the first forward, fixed across steps.
 
 Prints one JSON line: Mpix/s = 8 x 1024 x 1024 / (t_fwd + t_bwd), plus the roofline of the
-dominant kernel (HIP-event timing through the library's profiling hook) and the CPU oracle
-timed on a bounded sample of the same workload.
+dominant kernel (HIP-event timing through the library's profiling hook), the CPU oracle timed
+on a bounded sample of the same workload, and (unless --no-fit) the fit time-to-tolerance of
+the proposal-warm-started fit against the plain fit:
+
+  fit      35,002-vertex / 70,000-triangle template, 80-dim shape basis, 27 SH, 6-DoF pose;
+           16 subjects x 4 views at 512^2, masked L2, Adam lr 1e-2, 200 iterations (plain,
+           zero-initialised); the proposal network is trained first on renders generated on
+           the fly (--train-samples, default 200,000), then the warm-started fit runs until it
+           reaches the plain fit's final loss.  Reported: both wall-clock times and their ratio.
+
+--workload multi instead times the batched uneven workload (32 meshes, 2k-200k triangles,
+1-8 views at 256/512/1024^2) through the multi-mesh runtime: autotuned, one hipGraph launch
+per forward+backward step.
 """
 from __future__ import annotations
 
@@ -76,6 +87,90 @@ def cpu_baseline(scene, threads, views=1):
                        f"fwd {t1 - t0:.2f}s bwd {t3 - t2:.2f}s")
 
 
+def fit_experiment(train_samples):
+    """Proposal training on renders, then plain vs warm-started fit on held-out subjects."""
+    import numpy as np
+    import torch
+    from gvv_amd import fit, proposal
+
+    dev = torch.device("cuda", 0)
+    cfg = fit.FitConfig()
+    model = fit.ShapeModel(cfg, dev)
+    net = proposal.ProposalNet(dev, seed=0)
+    log = proposal.train(net, model, samples=train_samples, batch=64, seed=1)
+    gt = fit.sample_params(np.random.Generator(np.random.PCG64(12345)), cfg.batch, cfg.n_basis, dev)
+    rep = proposal.compare_fits(net, model, gt)
+    return dict(metric="fit time-to-tolerance vs plain", ratio=None if rep.ratio is None else round(rep.ratio, 4),
+                plain_seconds=round(rep.plain_seconds, 4), plain_iters=rep.plain_iters,
+                plain_final_loss=rep.plain_final_loss, plain_initial_loss=rep.plain_initial_loss,
+                warm_seconds=None if rep.warm_seconds is None else round(rep.warm_seconds, 4),
+                warm_iters=rep.warm_iters, warm_initial_loss=rep.warm_initial_loss, fallbacks=rep.fallbacks,
+                train_samples=log.samples, train_seconds=round(log.seconds, 2),
+                config=dict(template_vertices=model.N, template_triangles=model.F, basis=cfg.n_basis,
+                            subjects=cfg.batch, views=cfg.views, resolution=cfg.resolution, adam_lr=cfg.lr))
+
+
+def bench_multi(args):
+    """Batched uneven workload through the multi-mesh runtime (graph replay, autotuned)."""
+    import numpy as np
+    import torch
+    from gvv_amd import scenes
+    from gvv_amd.multi import MultiRenderer
+
+    dev = torch.device("cuda", 0)
+    batch = scenes.config2()
+    mr = MultiRenderer(batch, device=dev)
+    rng = np.random.Generator(np.random.PCG64(7))
+    for b in mr.bufs:
+        b["g_render"].copy_(torch.from_numpy(rng.normal(size=tuple(b["g_render"].shape)).astype(np.float32)))
+
+    def timed(steps):
+        for _ in range(args.warmup):
+            mr.run(backward=True)
+        torch.cuda.synchronize()
+        a, z = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(steps):
+            mr.run(backward=True)
+        z.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(z) / steps
+
+    mr.use_graph(False)
+    ms_plain_default = timed(args.steps)
+    mr.use_graph(True)
+    ms_graph_default = timed(args.steps)
+    t0 = time.perf_counter()
+    mr.autotune(reps=3)
+    tune_s = time.perf_counter() - t0
+    ms = timed(args.steps)
+    P = mr.pixels()
+    tuning = [mr.tuning(i) for i in range(len(batch))]
+    line = {
+        "metric": "fwd+bwd Mpix/s, batched uneven meshes (one graph launch per step)",
+        "value": round(P / (ms / 1e3) / 1e6, 2),
+        "unit": "Mpix/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (32 seeded icosphere/grid meshes)",
+        "config": {"workload": "multi32_uneven", "meshes": len(batch), "pixels": P,
+                   "triangle_views": mr.triangles(),
+                   "triangles_per_s": round(mr.triangles() / (ms / 1e3), 1),
+                   "ms_plain_launches_default_tuning": round(ms_plain_default, 4),
+                   "ms_graph_default_tuning": round(ms_graph_default, 4),
+                   "autotune_seconds": round(tune_s, 2),
+                   "tuning": [dict(mesh=s.meta["name"], tile=t, large_box=lb, size_class=c)
+                              for s, (t, lb, c) in zip(batch, tuning)]},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,6 +179,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-views", type=int, default=2)
     ap.add_argument("--kernels-json", default=None, help="also write per-kernel timings here")
+    ap.add_argument("--workload", choices=("render", "multi"), default="render")
+    ap.add_argument("--no-fit", action="store_true", help="skip the proposal + fit time-to-tolerance run")
+    ap.add_argument("--train-samples", type=int, default=200_000)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
                     help="after timing, run one plain device-to-device copy of MB MiB (reference kernel for "
                          "scaling rocprof FETCH_SIZE / WRITE_SIZE counters)")
@@ -94,6 +192,9 @@ def main():
               f"runs on a single MI355X; --gpus {args.gpus} (WORLD_SIZE={world}) is not supported.",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "multi":
+        return bench_multi(args)
 
     import numpy as np
     import torch
@@ -160,6 +261,15 @@ def main():
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         cpu = cpu_baseline(scene, threads, views=args.cpu_views)
 
+    fit = None
+    if not args.no_fit:
+        del rast
+        torch.cuda.synchronize()
+        try:
+            fit = fit_experiment(args.train_samples)
+        except Exception as e:      # reported in the line; the throughput figure stands on its own
+            fit = dict(error=f"{type(e).__name__}: {e}")
+
     line = {
         "metric": "fwd+bwd Mpix/s (8x1024^2 views, 100k tris)",
         "value": round(value, 2),
@@ -180,6 +290,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels_ms": {k: round(v[0] / v[1], 4) for k, v in kernels.items()},
+        "fit": fit,
     }
     if args.kernels_json:
         with open(args.kernels_json, "w") as fh:

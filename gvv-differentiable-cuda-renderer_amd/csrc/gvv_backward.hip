@@ -280,7 +280,9 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
   __shared__ float s_acc[kFaceSlots][NT1];
   __shared__ int s_face[kFaceSlots];
   const int lane = threadIdx.x;
-  const int tile = blockIdx.x, bc = blockIdx.y;
+  const int tiles = a.tiles_x * a.tiles_y;
+  const int64_t work = xcd_contiguous(blockIdx.x, gridDim.x);    // one (view, tile) per workgroup
+  const int tile = (int)(work % tiles), bc = (int)(work / tiles);
   const int b = bc / a.C;
   const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
   float sh_acc[27];
@@ -353,7 +355,6 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
   if (NT > 0 && nslots > 0) flush();
 
   if (SHADED) {
-    const int tiles = gridDim.x;
 #pragma unroll
     for (int k = 0; k < 27; ++k) {
       const float s = any ? wave_total(sh_acc[k]) : 0.f;
@@ -405,7 +406,7 @@ __global__ void __launch_bounds__(256) k_face_jac(const float* __restrict__ vpos
 
 template <int ALBEDO, bool SHADED, bool M2D, bool HAVE_BARY>
 static void launch_pixel_grad_t(const BackwardArgs& a, hipStream_t s) {
-  dim3 grid(a.tiles_x * a.tiles_y, a.B * a.C);
+  dim3 grid((unsigned)((int64_t)a.tiles_x * a.tiles_y * a.B * a.C));
   if (a.tile == 8) hipLaunchKernelGGL((k_pixel_grad<ALBEDO, SHADED, M2D, HAVE_BARY, 8>), grid, dim3(64), 0, s, a);
   else if (a.tile == 32) hipLaunchKernelGGL((k_pixel_grad<ALBEDO, SHADED, M2D, HAVE_BARY, 32>), grid, dim3(64), 0, s, a);
   else hipLaunchKernelGGL((k_pixel_grad<ALBEDO, SHADED, M2D, HAVE_BARY, 16>), grid, dim3(64), 0, s, a);

@@ -284,6 +284,26 @@ int gvv_backward(void* handle, int32_t B, int32_t texH, int32_t texW, const floa
   return 0;
 }
 
+int gvv_set_tuning(void* handle, int32_t tile, int32_t large_box) {
+  Handle* h = (Handle*)handle;
+  if (!h) return fail("null handle");
+  if (tile != 8 && tile != 16 && tile != 32) return fail("tile must be 8, 16 or 32");
+  if (large_box < 16 || large_box > (1 << 20)) return fail("large_box must be in [16, 2^20]");
+  std::lock_guard<std::mutex> lock(h->mu);
+  h->tile = tile;
+  h->large_box = large_box;
+  return 0;
+}
+
+int gvv_get_tuning(void* handle, int32_t* tile, int32_t* large_box) {
+  Handle* h = (Handle*)handle;
+  if (!h || !tile || !large_box) return fail("null pointer");
+  std::lock_guard<std::mutex> lock(h->mu);
+  *tile = h->tile;
+  *large_box = h->large_box;
+  return 0;
+}
+
 int gvv_profile_enable(void* handle, int32_t enable) {
   Handle* h = (Handle*)handle;
   if (!h) return fail("null handle");

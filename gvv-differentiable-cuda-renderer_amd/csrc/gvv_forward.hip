@@ -204,7 +204,7 @@ __global__ void __launch_bounds__(64) k_raster(const float* __restrict__ vpos, c
   __shared__ FaceStage st[64];
   __shared__ unsigned cand[kCandCap];
   const int lane = threadIdx.x;
-  const int64_t i = (int64_t)blockIdx.x * 64 + lane;
+  const int64_t i = xcd_contiguous(blockIdx.x, gridDim.x) * 64 + lane;
   const bool in_range = i < (int64_t)B * C * F;
   const int f = in_range ? (int)(i % F) : 0;
   const int bc = in_range ? (int)(i / F) : 0;
@@ -293,7 +293,7 @@ __global__ void k_zero_u32(unsigned int* p, int n) {
 
 template <int ALBEDO, bool SHADE>
 __global__ void __launch_bounds__(256) k_resolve(ResolveArgs a) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t i = xcd_contiguous(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   const int64_t P = (int64_t)a.W * a.H;
   if (i >= (int64_t)a.B * a.C * P) return;
   unsigned long long key = a.keys[i];

@@ -60,6 +60,17 @@ struct ProfScope {
   ~ProfScope() { if (p) p->end(s); }
 };
 
+// MI355X dispatches consecutive workgroups round-robin over its 8 XCDs, each with its own L2.
+// xcd_contiguous maps hardware workgroup id wg of n to a logical work index such that every
+// XCD receives one contiguous range of logical indices (a bijection on [0, n)); neighbouring
+// work -- pixels of one view, faces of one view -- then shares one L2 instead of eight.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int64_t xcd_contiguous(int64_t wg, int64_t n) {
+  const int64_t x = wg % kXcds, j = wg / kXcds;
+  const int64_t q = n / kXcds, r = n % kXcds;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + j;
+}
+
 // Albedo and shading modes, numbered as the reference enums
 // (cpp/src/Renderer/CUDABasedRasterizationInput.h:25-35).
 enum Albedo { kVertexColor = 0, kTextured = 1, kNormal = 2, kLighting = 3, kForegroundMask = 4 };

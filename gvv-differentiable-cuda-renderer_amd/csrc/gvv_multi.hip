@@ -1,0 +1,305 @@
+// gvv_multi.hip -- multi-mesh runtime: many meshes of uneven size in one hipGraph launch, with
+// an autotuner over the per-mesh launch settings (C ABI in include/gvv_multi.h).
+//
+// Each mesh keeps its own renderer handle (topology, CSR, scratch), so meshes share no
+// buffers and their kernel chains are independent.  A run plans every mesh on the host (this
+// is where scratch grows, outside any capture), then either launches the chains in order or
+// captures them once into a hipGraph whose branches -- one per hardware queue -- take the
+// meshes longest-first (LPT), so small meshes run beside large ones instead of after them.
+// The graph is replayed while the io table and the settings are unchanged.
+//
+// The reference renders one topology per op and serialises the batch on the host
+// (cpp/src/TensorflowOperators/CudaRenderer/CudaRenderer.cpp:23, :309-328); nothing here has a
+// counterpart there.
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "gvv_handle.hpp"
+#include "../../include/gvv_multi.h"
+#include "../../include/gvv_renderer.h"
+
+namespace {
+
+using namespace gvv;
+
+constexpr int kLanes = 4;                       // graph branches: the hardware queues per process
+constexpr int kTiles[3] = {8, 16, 32};          // backward pixel-tile edges
+constexpr int kLargeBoxes[3] = {64, 256, 1024}; // raster large-face thresholds (pixels)
+
+int fail(const std::string& m) { return set_last_error(m); }
+
+#define MULTI_CHECK(expr)                                                            \
+  do {                                                                               \
+    hipError_t _e = (expr);                                                          \
+    if (_e != hipSuccess) return fail(std::string(#expr) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+// Size class: triangle-count bucket (<8k, <32k, <128k, larger) x pixels per view
+// (<=256^2, <=512^2, larger).
+int size_class(const Handle* h) {
+  const int tb = h->F < 8192 ? 0 : h->F < 32768 ? 1 : h->F < 131072 ? 2 : 3;
+  const int64_t px = (int64_t)h->W * h->H;
+  const int pb = px <= 256 * 256 ? 0 : px <= 512 * 512 ? 1 : 2;
+  return tb * 3 + pb;
+}
+
+struct Multi {
+  std::vector<Handle*> meshes;
+  std::vector<int> cls;
+  int albedo = 0, shading = 0, device = 0;
+  bool use_graph = true;
+  hipStream_t lanes[kLanes] = {};
+  hipEvent_t fork = nullptr;
+  hipEvent_t join[kLanes] = {};
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<char> key;      // what the graph was captured for
+  std::mutex mu;
+
+  void drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    key.clear();
+  }
+  ~Multi() {
+    drop_graph();
+    for (auto s : lanes)
+      if (s) (void)hipStreamDestroy(s);
+    if (fork) (void)hipEventDestroy(fork);
+    for (auto e : join)
+      if (e) (void)hipEventDestroy(e);
+    for (auto* h : meshes) gvv_destroy(h);
+  }
+};
+
+double mesh_cost(const Handle* h, int B) {
+  return (double)B * h->C * ((double)h->F + 0.5 * (double)h->W * h->H);
+}
+
+int plan_mesh(Multi* m, int i, const gvv_mesh_io& io, bool backward, hipStream_t s, ForwardPlan& fp, BackwardPlan& bp) {
+  Handle* h = m->meshes[i];
+  const std::string where = "mesh " + std::to_string(i) + ": ";
+  if (plan_forward(h, io.batch, io.tex_h, io.tex_w, io.vpos, io.vcol, io.tex, io.sh, io.extr, io.intr, io.bary,
+                   io.face, io.render, io.vnormal, nullptr, s, fp))
+    return fail(where + gvv_last_error());
+  if (backward) {
+    if (!io.g_render) return fail(where + "g_render is null");
+    if (plan_backward(h, io.batch, io.tex_h, io.tex_w, io.g_render, nullptr, nullptr, io.vpos, io.vcol, io.tex, io.sh,
+                      io.vnormal, nullptr, io.face, io.extr, io.intr, io.g_pos, io.g_col, io.g_tex, io.g_sh, bp))
+      return fail(where + gvv_last_error());
+  }
+  return 0;
+}
+
+hipError_t enqueue_mesh(const ForwardPlan& fp, const BackwardPlan& bp, bool backward, hipStream_t s) {
+  hipError_t e = launch_forward(fp, s);
+  if (e == hipSuccess && backward) e = launch_backward(bp, s);
+  return e;
+}
+
+int check_device(const Multi* m) {
+  int dev = -1;
+  MULTI_CHECK(hipGetDevice(&dev));
+  if (dev != m->device) return fail("runtime was created on device " + std::to_string(m->device));
+  return 0;
+}
+
+int plan_all(Multi* m, const gvv_mesh_io* io, bool backward, hipStream_t s, std::vector<ForwardPlan>& fp,
+             std::vector<BackwardPlan>& bp) {
+  const int n = (int)m->meshes.size();
+  fp.resize(n);
+  bp.resize(n);
+  for (int i = 0; i < n; ++i)
+    if (plan_mesh(m, i, io[i], backward, s, fp[i], bp[i])) return 1;
+  return 0;
+}
+
+std::vector<char> graph_key(const Multi* m, const gvv_mesh_io* io, bool backward) {
+  const size_t n = m->meshes.size();
+  std::vector<char> k(n * sizeof(gvv_mesh_io) + 1 + n * 2 * sizeof(int));
+  memcpy(k.data(), io, n * sizeof(gvv_mesh_io));
+  k[n * sizeof(gvv_mesh_io)] = backward ? 1 : 0;
+  int* t = reinterpret_cast<int*>(k.data() + n * sizeof(gvv_mesh_io) + 1);
+  for (size_t i = 0; i < n; ++i) {
+    memcpy(t + 2 * i, &m->meshes[i]->tile, sizeof(int));
+    memcpy(t + 2 * i + 1, &m->meshes[i]->large_box, sizeof(int));
+  }
+  return k;
+}
+
+// Captures every mesh's chain into one graph: fork from lane 0, meshes longest-first onto the
+// least-loaded lane, join back into lane 0.
+int capture(Multi* m, const gvv_mesh_io* io, bool backward, const std::vector<ForwardPlan>& fp,
+            const std::vector<BackwardPlan>& bp) {
+  const int n = (int)m->meshes.size();
+  std::vector<int> order(n);
+  for (int i = 0; i < n; ++i) order[i] = i;
+  std::sort(order.begin(), order.end(), [&](int a, int b) {
+    return mesh_cost(m->meshes[a], io[a].batch) > mesh_cost(m->meshes[b], io[b].batch);
+  });
+  m->drop_graph();
+  MULTI_CHECK(hipStreamBeginCapture(m->lanes[0], hipStreamCaptureModeThreadLocal));
+  hipError_t e = hipEventRecord(m->fork, m->lanes[0]);
+  for (int k = 1; k < kLanes && e == hipSuccess; ++k) e = hipStreamWaitEvent(m->lanes[k], m->fork, 0);
+  double load[kLanes] = {0.0, 0.0, 0.0, 0.0};
+  for (int j = 0; j < n && e == hipSuccess; ++j) {
+    const int i = order[j];
+    int lane = 0;
+    for (int k = 1; k < kLanes; ++k)
+      if (load[k] < load[lane]) lane = k;
+    load[lane] += mesh_cost(m->meshes[i], io[i].batch);
+    e = enqueue_mesh(fp[i], bp[i], backward, m->lanes[lane]);
+  }
+  for (int k = 1; k < kLanes && e == hipSuccess; ++k) {
+    e = hipEventRecord(m->join[k], m->lanes[k]);
+    if (e == hipSuccess) e = hipStreamWaitEvent(m->lanes[0], m->join[k], 0);
+  }
+  hipGraph_t g = nullptr;
+  const hipError_t e_end = hipStreamEndCapture(m->lanes[0], &g);
+  if (e != hipSuccess || e_end != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return fail(std::string("graph capture: ") + hipGetErrorString(e != hipSuccess ? e : e_end));
+  }
+  m->graph = g;
+  MULTI_CHECK(hipGraphInstantiate(&m->exec, g, nullptr, nullptr, 0));
+  m->key = graph_key(m, io, backward);
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gvv_multi_create(const gvv_mesh_desc* meshes, int32_t n, int32_t albedo_mode, int32_t shading_mode,
+                     void** handle) {
+  if (!handle) return fail("handle pointer is null");
+  *handle = nullptr;
+  if (!meshes || n <= 0) return fail("need at least one mesh");
+  if (albedo_mode != kVertexColor && albedo_mode != kTextured && albedo_mode != kForegroundMask)
+    return fail("multi-mesh runtime supports the vertexColor, textured and foregroundMask modes");
+  Multi* m = new Multi();
+  m->albedo = albedo_mode;
+  m->shading = shading_mode;
+  for (int i = 0; i < n; ++i) {
+    const gvv_mesh_desc& d = meshes[i];
+    void* h = nullptr;
+    if (gvv_create(d.faces, d.n_face_idx, d.texcoords, d.n_face_idx * 2, d.n_vertices, d.n_cameras, d.width,
+                   d.height, albedo_mode, shading_mode, 1, 1, 0, &h)) {
+      const std::string msg = "mesh " + std::to_string(i) + ": " + gvv_last_error();
+      delete m;
+      return fail(msg);
+    }
+    m->meshes.push_back((Handle*)h);
+    m->cls.push_back(size_class((Handle*)h));
+  }
+  hipError_t e = hipGetDevice(&m->device);
+  for (int k = 0; k < kLanes && e == hipSuccess; ++k) e = hipStreamCreateWithFlags(&m->lanes[k], hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&m->fork, hipEventDisableTiming);
+  for (int k = 0; k < kLanes && e == hipSuccess; ++k) e = hipEventCreateWithFlags(&m->join[k], hipEventDisableTiming);
+  if (e != hipSuccess) {
+    delete m;
+    return fail(std::string("gvv_multi_create: ") + hipGetErrorString(e));
+  }
+  *handle = m;
+  return 0;
+}
+
+void gvv_multi_destroy(void* handle) { delete (Multi*)handle; }
+
+int gvv_multi_use_graph(void* handle, int32_t on) {
+  Multi* m = (Multi*)handle;
+  if (!m) return fail("null handle");
+  std::lock_guard<std::mutex> lock(m->mu);
+  m->use_graph = on != 0;
+  return 0;
+}
+
+int gvv_multi_run(void* handle, const gvv_mesh_io* io, int32_t backward, void* stream) {
+  Multi* m = (Multi*)handle;
+  if (!m) return fail("null handle");
+  if (!io) return fail("null io table");
+  std::lock_guard<std::mutex> lock(m->mu);
+  if (check_device(m)) return 1;
+  hipStream_t s = (hipStream_t)stream;
+  std::vector<ForwardPlan> fp;
+  std::vector<BackwardPlan> bp;
+  if (plan_all(m, io, backward != 0, s, fp, bp)) return 1;
+  if (!m->use_graph) {
+    for (size_t i = 0; i < fp.size(); ++i) MULTI_CHECK(enqueue_mesh(fp[i], bp[i], backward != 0, s));
+    return 0;
+  }
+  if (!m->exec || m->key != graph_key(m, io, backward != 0))
+    if (capture(m, io, backward != 0, fp, bp)) return 1;
+  MULTI_CHECK(hipGraphLaunch(m->exec, s));
+  return 0;
+}
+
+int gvv_multi_autotune(void* handle, const gvv_mesh_io* io, int32_t reps, void* stream) {
+  Multi* m = (Multi*)handle;
+  if (!m) return fail("null handle");
+  if (!io) return fail("null io table");
+  if (reps <= 0) return fail("reps must be > 0");
+  std::lock_guard<std::mutex> lock(m->mu);
+  if (check_device(m)) return 1;
+  hipStream_t s = (hipStream_t)stream;
+  hipEvent_t a = nullptr, b = nullptr;
+  MULTI_CHECK(hipEventCreate(&a));
+  MULTI_CHECK(hipEventCreate(&b));
+  int rc = 0;
+  const int n = (int)m->meshes.size();
+  for (int c = 0; c < 12 && rc == 0; ++c) {
+    int rep = -1;
+    for (int i = 0; i < n; ++i)
+      if (m->cls[i] == c && (rep < 0 || mesh_cost(m->meshes[i], io[i].batch) > mesh_cost(m->meshes[rep], io[rep].batch)))
+        rep = i;
+    if (rep < 0) continue;
+    Handle* h = m->meshes[rep];
+    float best = 1e30f;
+    int best_tile = h->tile, best_box = h->large_box;
+    for (int ti = 0; ti < 3 && rc == 0; ++ti)
+      for (int bi = 0; bi < 3 && rc == 0; ++bi) {
+        h->tile = kTiles[ti];
+        h->large_box = kLargeBoxes[bi];
+        ForwardPlan fp;
+        BackwardPlan bp;
+        if (plan_mesh(m, rep, io[rep], true, s, fp, bp)) { rc = 1; break; }
+        hipError_t e = enqueue_mesh(fp, bp, true, s);   // warm-up
+        float t_min = 1e30f;
+        for (int r = 0; r < reps && e == hipSuccess; ++r) {
+          e = hipEventRecord(a, s);
+          if (e == hipSuccess) e = enqueue_mesh(fp, bp, true, s);
+          if (e == hipSuccess) e = hipEventRecord(b, s);
+          if (e == hipSuccess) e = hipEventSynchronize(b);
+          float ms = 0.f;
+          if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+          if (e == hipSuccess) t_min = std::min(t_min, ms);
+        }
+        if (e != hipSuccess) { rc = fail(std::string("autotune: ") + hipGetErrorString(e)); break; }
+        if (t_min < best) { best = t_min; best_tile = kTiles[ti]; best_box = kLargeBoxes[bi]; }
+      }
+    for (int i = 0; i < n; ++i)
+      if (m->cls[i] == c) { m->meshes[i]->tile = best_tile; m->meshes[i]->large_box = best_box; }
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  m->drop_graph();
+  return rc;
+}
+
+int gvv_multi_tuning(void* handle, int32_t mesh, int32_t* tile, int32_t* large_box, int32_t* size_class) {
+  Multi* m = (Multi*)handle;
+  if (!m) return fail("null handle");
+  std::lock_guard<std::mutex> lock(m->mu);
+  if (mesh < 0 || mesh >= (int32_t)m->meshes.size()) return fail("mesh index out of range");
+  if (tile) *tile = m->meshes[mesh]->tile;
+  if (large_box) *large_box = m->meshes[mesh]->large_box;
+  if (size_class) *size_class = m->cls[mesh];
+  return 0;
+}
+
+}  // extern "C"

@@ -1,5 +1,5 @@
-"""ctypes binding of libgvv_amd.so (C ABI declared in include/gvv_renderer.h and
-include/gvv_proposal.h).
+"""ctypes binding of libgvv_amd.so (C ABI declared in include/gvv_renderer.h,
+include/gvv_multi.h and include/gvv_proposal.h).
 
 The library is the only compute path: if it has not been built this module raises; there is
 no CPU fallback.
@@ -13,9 +13,11 @@ _PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.environ.get("GVV_AMD_LIB", os.path.join(_PKG, "libgvv_amd.so"))
 
 EXPORTS = ("gvv_last_error", "gvv_version", "gvv_create", "gvv_destroy", "gvv_forward", "gvv_backward",
-           "gvv_profile_enable", "gvv_profile_report",
+           "gvv_profile_enable", "gvv_profile_report", "gvv_set_tuning", "gvv_get_tuning",
            "gvv_conv_out_size", "gvv_conv_forward", "gvv_conv_backward_data", "gvv_conv_workspace_floats",
-           "gvv_conv_backward_weight", "gvv_conv_prepare_weights", "gvv_pack_mosaic")
+           "gvv_conv_backward_weight", "gvv_conv_prepare_weights", "gvv_pack_mosaic",
+           "gvv_multi_create", "gvv_multi_destroy", "gvv_multi_run", "gvv_multi_use_graph", "gvv_multi_autotune",
+           "gvv_multi_tuning")
 
 _lib = None
 
@@ -48,6 +50,10 @@ def load():
     L.gvv_profile_enable.restype = ctypes.c_int
     L.gvv_profile_report.argtypes = [P, ctypes.c_char_p, i64]
     L.gvv_profile_report.restype = i64
+    L.gvv_set_tuning.argtypes = [P, i32, i32]
+    L.gvv_set_tuning.restype = ctypes.c_int
+    L.gvv_get_tuning.argtypes = [P, ctypes.POINTER(i32), ctypes.POINTER(i32)]
+    L.gvv_get_tuning.restype = ctypes.c_int
     L.gvv_conv_out_size.argtypes = [i32] * 4
     L.gvv_conv_out_size.restype = ctypes.c_int
     L.gvv_conv_forward.argtypes = [i32] * 8 + [P, P, P, i32, P, i32, P]
@@ -62,6 +68,18 @@ def load():
     L.gvv_conv_prepare_weights.restype = ctypes.c_int
     L.gvv_pack_mosaic.argtypes = [P, i32, i32, i32, i32, i32, i32, P, P]
     L.gvv_pack_mosaic.restype = ctypes.c_int
+    L.gvv_multi_create.argtypes = [P, i32, i32, i32, ctypes.POINTER(P)]
+    L.gvv_multi_create.restype = ctypes.c_int
+    L.gvv_multi_destroy.argtypes = [P]
+    L.gvv_multi_destroy.restype = None
+    L.gvv_multi_run.argtypes = [P, P, i32, P]
+    L.gvv_multi_run.restype = ctypes.c_int
+    L.gvv_multi_use_graph.argtypes = [P, i32]
+    L.gvv_multi_use_graph.restype = ctypes.c_int
+    L.gvv_multi_autotune.argtypes = [P, P, i32, P]
+    L.gvv_multi_autotune.restype = ctypes.c_int
+    L.gvv_multi_tuning.argtypes = [P, i32, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32)]
+    L.gvv_multi_tuning.restype = ctypes.c_int
     _lib = L
     return L
 

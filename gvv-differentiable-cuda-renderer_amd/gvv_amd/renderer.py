@@ -157,6 +157,15 @@ class Rasterizer:
         _lib.check(rc, "gvv_backward")
         return g_pos, g_col, g_tex, g_sh
 
+    def set_tuning(self, tile: int = 16, large_box: int = 256):
+        """Backward pixel-tile edge (8/16/32) and the raster's large-face threshold in pixels."""
+        _lib.check(_lib.load().gvv_set_tuning(self._h, int(tile), int(large_box)), "gvv_set_tuning")
+
+    def tuning(self):
+        t, lb = ctypes.c_int32(), ctypes.c_int32()
+        _lib.check(_lib.load().gvv_get_tuning(self._h, ctypes.byref(t), ctypes.byref(lb)), "gvv_get_tuning")
+        return t.value, lb.value
+
     def profile(self, enable: bool = True):
         """Enable/disable per-kernel HIP-event timing of this handle (clears the figures)."""
         _lib.check(_lib.load().gvv_profile_enable(self._h, int(enable)), "gvv_profile_enable")

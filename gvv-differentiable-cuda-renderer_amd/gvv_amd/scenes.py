@@ -260,6 +260,44 @@ def config1(seed=0, width=1024, height=1024, cols=250, rows=200, n_cameras=8, te
                  meta=dict(name=f"cylinder{f.shape[0]}_{n_cameras}x{width}", seed=seed))
 
 
+def config2(seed=0, n_meshes=32, tri_range=(2000, 200000), resolutions=(256, 512, 1024), max_views=8):
+    """Batched uneven workload: n_meshes meshes with triangle counts log-uniform in tri_range,
+    alternately icospheres (the subdivision level whose count is nearest in log scale, within
+    the range) and capped-cylinder grids (cols x rows chosen for the drawn count); each with
+    1..max_views views (uniform) at one square resolution drawn from `resolutions`, its own
+    vertex colours, SH and cameras on a 3 m sphere as in config0.  vertexColor + shaded."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    lo, hi = np.log(tri_range[0]), np.log(tri_range[1])
+    levels = [L for L in range(1, 9) if tri_range[0] <= 20 * 4 ** L <= tri_range[1]]
+    for k in range(n_meshes):
+        r_geo, r_col, r_sh, r_cam, r_pick = rng.spawn(5)
+        F_target = float(np.exp(r_pick.uniform(lo, hi)))
+        C = int(r_pick.integers(1, max_views + 1))
+        R = int(resolutions[r_pick.integers(0, len(resolutions))])
+        if k % 2 == 0 and levels:
+            L = min(levels, key=lambda L: abs(np.log(20 * 4 ** L) - np.log(F_target)))
+            v, f = icosphere(L)
+            v = v + r_geo.normal(0.0, 0.01, size=v.shape)
+            uv = np.zeros((f.shape[0], 3, 2), np.float32)
+            kind = f"icosphere{f.shape[0]}"
+        else:
+            cols = max(3, int(round(np.sqrt(F_target * 1.25 / 2.0))))
+            rows = max(2, int(round(F_target / (2.0 * cols))))
+            v, f, uv = capped_cylinder(cols, rows, 0.8, 1.6, r_geo, 0.04)
+            v = v - v.mean(0)
+            kind = f"grid{f.shape[0]}"
+        N = v.shape[0]
+        E, K = sphere_cameras(C, 3.0, R, R, r_cam)
+        out.append(Scene(faces=f, uv=uv, vpos=(v * MM).astype(np.float32)[None],
+                         vcol=r_col.uniform(0.0, 1.0, size=(1, N, 3)).astype(np.float32),
+                         tex=np.full((1, 1, 1, 3), 0.5, np.float32), sh=sh_coefficients(C, r_sh)[None],
+                         extr=E[None], intr=K[None], target=np.zeros((1, C, R, R, 3), np.float32), width=R, height=R,
+                         albedo_mode="vertexColor", shading_mode="shaded",
+                         meta=dict(name=f"{kind}_{C}x{R}", seed=seed, index=k)))
+    return out
+
+
 def single_triangle(width=64, height=64, z_m=2.0, albedo_mode="vertexColor", shading_mode="shaded",
                     tri_m=((-0.5, -0.4, 0.0), (0.5, -0.4, 0.0), (0.0, 0.6, 0.0))):
     """One triangle in the plane z=0 seen by a camera on the -z axis looking at +z."""

@@ -86,6 +86,15 @@ int gvv_backward(void* handle, int32_t B, int32_t texH, int32_t texW, const floa
                  const float* vnormal, const float* bary, const int32_t* face, const float* extr,
                  const float* intr, float* g_pos, float* g_col, float* g_tex, float* g_sh, void* hip_stream);
 
+/* Launch settings of one handle (no counterpart in the reference; chosen by the autotuner of
+ * the multi-mesh runtime, gvv_multi.h).  tile: pixel tile edge of the backward pixel kernel,
+ * 8, 16 or 32 (default 16).  large_box: bounding-box size in pixels above which the raster
+ * kernel hands a face to its persistent large-face queue, 16 .. 2^20 (default 256).  The
+ * forward results do not depend on either; backward results change only in the order of
+ * floating-point additions. */
+int gvv_set_tuning(void* handle, int32_t tile, int32_t large_box);
+int gvv_get_tuning(void* handle, int32_t* tile, int32_t* large_box);
+
 /* Kernel timing (no counterpart in the reference, which has no timing code).  When enabled,
  * every kernel launch of this handle is bracketed by HIP events on its stream; enabling or
  * disabling clears the accumulated figures. */

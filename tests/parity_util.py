@@ -14,12 +14,15 @@ def to_dev(a, dev):
     return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
 
 
-def run_parity(scene, device, albedo, shading, m2d, seed=0, image_fs=1, use_bary=False, check_grads=True):
+def run_parity(scene, device, albedo, shading, m2d, seed=0, image_fs=1, use_bary=False, check_grads=True, tuning=None):
     kw = scene.renderer_kwargs()
     kw.update(albedo_mode=albedo, shading_mode=shading)
     orc = O.Oracle(**kw, image_filter_size=image_fs)
     ref = orc.forward(*scene.inputs())
     rast = Rasterizer(**kw, image_filter_size=image_fs)
+    if tuning is not None:
+        rast.set_tuning(*tuning)
+        assert rast.tuning() == tuple(tuning)
     ins = [to_dev(a, device) for a in scene.inputs()]
     bary, face, render, vnormal, _ = rast.forward(*ins)
     torch.cuda.synchronize()

@@ -1,5 +1,5 @@
-"""The C ABI without a GPU: libgvv_amd.so loads, exports every function include/gvv_renderer.h
-and include/gvv_proposal.h declare, and rejects invalid arguments on the host before any
+"""The C ABI without a GPU: libgvv_amd.so loads, exports every function include/gvv_renderer.h,
+include/gvv_multi.h and include/gvv_proposal.h declare, and rejects invalid arguments on the host before any
 device call."""
 import ctypes
 import os
@@ -12,7 +12,7 @@ from gvv_amd import _lib
 from gvv_amd.renderer import Rasterizer
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADERS = [os.path.join(ROOT, "include", h) for h in ("gvv_renderer.h", "gvv_proposal.h")]
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("gvv_renderer.h", "gvv_multi.h", "gvv_proposal.h")]
 
 
 def declared_functions():
@@ -90,3 +90,24 @@ def test_conv_rejects_invalid_geometry():
     rc = L.gvv_conv_backward_weight(2, 8, 8, 8, 32, 3, 2, 1, P, P, P, P, P, need - 1, None)
     assert rc != 0 and "workspace" in L.gvv_last_error().decode()
     assert L.gvv_conv_workspace_floats(2, 8, 8, 7, 32, 3, 2, 1) == -1
+
+
+def test_multi_structs_match_the_c_layout():
+    """gvv_mesh_desc / gvv_mesh_io as declared in include/gvv_multi.h (natural alignment)."""
+    from gvv_amd.multi import MeshDesc, MeshIO
+    assert ctypes.sizeof(MeshDesc) == 8 + 8 + 8 + 4 * 4
+    assert ctypes.sizeof(MeshIO) == 3 * 4 + 4 + 15 * 8
+    assert MeshIO.vpos.offset == 16 and MeshIO.g_sh.offset == 16 + 14 * 8
+    text = open(os.path.join(ROOT, "include", "gvv_multi.h")).read()
+    io_fields = re.findall(r"\*\s*(\w+);", text[text.index("typedef struct gvv_mesh_io"):])
+    assert io_fields[:15] == [f for f, _ in MeshIO._fields_[3:]]
+
+
+def test_multi_rejects_modes_without_gradients():
+    L = _lib.load()
+    h = ctypes.c_void_p()
+    from gvv_amd.multi import MeshDesc
+    d = (MeshDesc * 1)()
+    assert L.gvv_multi_create(d, 1, 2, 0, ctypes.byref(h)) != 0
+    assert "vertexColor, textured and foregroundMask" in L.gvv_last_error().decode()
+    assert L.gvv_multi_create(d, 0, 0, 0, ctypes.byref(h)) != 0

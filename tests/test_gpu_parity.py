@@ -71,3 +71,14 @@ def test_stress_visibility(device, seed):
     """Camera-plane crossings, near-full-screen, edge-on, sub-pixel and duplicate faces."""
     s = scenes.stress_soup(seed)
     _run(s, device, "vertexColor" if seed % 2 == 0 else "textured", "shaded", m2d=seed == 3, check_grads=seed < 2)
+
+
+@pytest.mark.parametrize("tuning", [(8, 64), (32, 1024), (8, 16), (32, 256)])
+def test_tuning_settings(device, tuning):
+    """Every backward tile edge and large-face threshold the autotuner may pick keeps the
+    forward bit-exact and the gradients within tolerance."""
+    s = scenes.config0(width=160, height=120, level=3)
+    s.tex = scenes.smooth_texture(np.random.Generator(np.random.PCG64(3)), 64, 48)[None]
+    s.uv = np.random.Generator(np.random.PCG64(4)).uniform(0, 1, size=s.uv.shape).astype(np.float32)
+    _run(s, device, "vertexColor", "shaded", m2d=True, tuning=tuning)
+    _run(s, device, "textured", "shaded", m2d=False, tuning=tuning)

@@ -73,3 +73,15 @@ def test_look_at_convention():
     assert np.allclose(R @ R.T, np.eye(3), atol=1e-6)
     assert np.allclose(R @ np.array([0.0, 0.0, 0.0]) + t, [0, 0, 2000.0])      # origin 2 m ahead (+z)
     assert (R @ np.array([0.0, 1000.0, 0.0]))[1] < 0                           # world up is image up (-y)
+
+
+def test_config2_batch_shape():
+    batch = scenes.config2(seed=0)
+    assert len(batch) == 32
+    for s in batch:
+        assert 2000 * 0.8 <= s.F <= 200000 * 1.25, s.meta
+        assert 1 <= s.C <= 8 and s.width == s.height and s.width in (256, 512, 1024)
+        assert s.vpos.shape == (1, s.N, 3) and s.sh.shape == (1, s.C, 27)
+        assert s.faces.min() >= 0 and s.faces.max() < s.N
+    Fs = sorted(s.F for s in batch)
+    assert Fs[-1] / Fs[0] > 10          # genuinely uneven
