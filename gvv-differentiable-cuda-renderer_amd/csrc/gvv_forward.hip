@@ -193,6 +193,10 @@ __device__ __forceinline__ EdgeTest readlane_edges(const EdgeTest& e, int j) {
   return r;
 }
 
+// Boxes up to this many pixels are walked by their own lane (all lanes at once); larger
+// staged boxes are walked by the whole wavefront, one face at a time.
+constexpr int kTinyBox = 16;
+
 // One wavefront per 64 consecutive (b, c, face) items.  Faces with bounding boxes up to
 // large_box pixels are staged and rasterised here (pre-test, compaction, exact test); larger
 // ones are appended to `large` for k_raster_large.
@@ -211,7 +215,7 @@ __global__ void __launch_bounds__(64) k_raster(const float* __restrict__ vpos, c
   bool small = false;
   EdgeTest et;
   et.on = false;
-  int bw = 0, bh = 0;
+  int bw = 0, bh = 0, fx0 = 0, fy0 = 0;
   if (in_range) {
     const RasterFace r = load_face(faces, proj + (size_t)bc * N, f, W, H);
     if (r.bx.x0 <= r.bx.x1 && r.bx.y0 <= r.bx.y1) {
@@ -229,15 +233,39 @@ __global__ void __launch_bounds__(64) k_raster(const float* __restrict__ vpos, c
         s.f = f; s.x0 = r.bx.x0; s.y0 = r.bx.y0; s.bw = bw; s.bc = bc;
         st[lane] = s;
         et = edge_test(r.p0, r.p1, r.p2);
+        fx0 = r.bx.x0;
+        fy0 = r.bx.y0;
       }
     }
   }
-  uint64_t todo = __ballot(small);
-  if (!todo) return;
+  if (!__ballot(small)) return;
   const int bc0 = __builtin_amdgcn_readfirstlane(bc);
   const bool one_view = __ballot(bc != bc0 && in_range) == 0;
   __syncthreads();
   int count = 0;
+  // tiny boxes: each lane walks its own box; pixel p of every lane's face per iteration
+  const int my_area = small && bw * bh <= kTinyBox ? bw * bh : 0;
+  for (int p = 0; __ballot(p < my_area); ++p) {
+    if (count > kCandCap - 64) {
+      __syncthreads();
+      flush_candidates(st, cand, count, cams, one_view, bc0, keys, W, H);
+      __syncthreads();
+      count = 0;
+    }
+    bool pass = false;
+    if (p < my_area) {
+      const int dy = p / bw;
+      pass = et.pass(fx0 + (p - dy * bw) + 0.5f, fy0 + dy + 0.5f);
+    }
+    const uint64_t m = __ballot(pass);
+    if (pass) {
+      const int off = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+      cand[count + off] = ((unsigned)lane << 24) | (unsigned)p;
+    }
+    count += __popcll(m);
+  }
+  // larger staged boxes: the whole wavefront walks one box at a time
+  uint64_t todo = __ballot(small && my_area == 0);
   while (todo) {
     const int j = __ffsll((unsigned long long)todo) - 1;
     todo &= todo - 1;
@@ -250,9 +278,32 @@ __global__ void __launch_bounds__(64) k_raster(const float* __restrict__ vpos, c
   flush_candidates(st, cand, count, cams, one_view, bc0, keys, W, H);
 }
 
+// Conservative x-range [xl, xr] of bounding-box row y (pixel centres at y + 0.5) inside the
+// three half-planes of the edge pre-test, widened by one pixel on each side so that float
+// rounding of the bounds never drops a pixel the pre-test would pass.  Without a usable edge
+// test the whole row is returned.
+__device__ __forceinline__ void row_span(const EdgeTest& e, int y, int x0, int x1, int& xl, int& xr) {
+  xl = x0;
+  xr = x1;
+  if (!e.on) return;
+  const float yc = y + 0.5f;
+  float lo = (float)x0, hi = (float)x1;    // range of pixel indices x (centre x + 0.5)
+  for (int i = 0; i < 3; ++i) {
+    const float rhs = -(e.b[i] * yc + e.c[i]);      // need a * xc >= rhs
+    if (e.a[i] > 0.f) lo = fmaxf(lo, rhs / e.a[i] - 1.5f);
+    else if (e.a[i] < 0.f) hi = fminf(hi, rhs / e.a[i] + 0.5f);
+    else if (rhs > 1.f) hi = lo - 1.f;              // horizontal edge with the row well outside
+  }
+  if (!(lo <= hi)) { xr = x0 - 1; return; }         // empty (also catches NaN)
+  xl = max(x0, (int)floorf(lo));
+  xr = min(x1, (int)ceilf(hi));
+}
+
 // Persistent wavefronts drain the large-face queue: each takes one face at a time from an
-// atomic cursor and runs the same pre-test / compaction / exact-test passes over its box.
-// Every wavefront leaves when the cursor passes the queue length written by k_raster.
+// atomic cursor.  Instead of the whole bounding box the wavefront walks, 64 rows at a time, the
+// conservative span of each row (row_span; a wave prefix sum turns spans into pixel indices),
+// applies the edge pre-test, compacts and runs the exact test as k_raster does.  Every
+// wavefront leaves when the cursor passes the queue length written by k_raster.
 __global__ void __launch_bounds__(64) k_raster_large(const float* __restrict__ vpos, const int32_t* __restrict__ faces,
                                                      const float4* __restrict__ proj, const Camera* __restrict__ cams,
                                                      unsigned long long* __restrict__ keys,
@@ -260,6 +311,7 @@ __global__ void __launch_bounds__(64) k_raster_large(const float* __restrict__ v
                                                      int C, int N, int W, int H) {
   __shared__ FaceStage st[1];
   __shared__ unsigned cand[kCandCap];
+  __shared__ int s_xl[64], s_start[64];
   const unsigned int n_large = counters[0];
   for (;;) {
     unsigned int item = 0;
@@ -281,7 +333,54 @@ __global__ void __launch_bounds__(64) k_raster_large(const float* __restrict__ v
     const EdgeTest et = edge_test(r.p0, r.p1, r.p2);
     __syncthreads();
     int count = 0;
-    collect_face(et, 0, r.bx.x0, r.bx.y0, bw, bh, st, cand, count, cams, true, bc, keys, W, H);
+    const int lane = threadIdx.x;
+    for (int rbase = 0; rbase < bh; rbase += 64) {
+      // this lane's row: span and length; inclusive scan of the lengths over the wave
+      const int y = r.bx.y0 + rbase + lane;
+      int xl = 0, xr = -1;
+      if (rbase + lane < bh) row_span(et, y, r.bx.x0, r.bx.x1, xl, xr);
+      const int len = xr >= xl ? xr - xl + 1 : 0;
+      int incl = len;
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const int t = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += t;
+      }
+      const int total = __shfl(incl, 63, 64);
+      if (total == 0) continue;
+      __syncthreads();
+      s_xl[lane] = xl;
+      s_start[lane] = incl - len;
+      __syncthreads();
+      for (int q0 = 0; q0 < total; q0 += 64) {
+        if (count > kCandCap - 64) {
+          __syncthreads();
+          flush_candidates(st, cand, count, cams, true, bc, keys, W, H);
+          __syncthreads();
+          count = 0;
+        }
+        const int q = q0 + lane;
+        bool pass = false;
+        int p = 0;
+        if (q < total) {
+          // the row holding span pixel q: the last row whose start is <= q
+          int row = 0;
+#pragma unroll
+          for (int step = 32; step > 0; step >>= 1)
+            if (row + step < 64 && s_start[row + step] <= q) row += step;
+          const int x = s_xl[row] + (q - s_start[row]);
+          const int yy = r.bx.y0 + rbase + row;
+          pass = et.pass(x + 0.5f, yy + 0.5f);
+          p = (yy - r.bx.y0) * bw + (x - r.bx.x0);
+        }
+        const uint64_t m = __ballot(pass);
+        if (pass) {
+          const int off = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+          cand[count + off] = (unsigned)p;
+        }
+        count += __popcll(m);
+      }
+    }
     __syncthreads();
     flush_candidates(st, cand, count, cams, true, bc, keys, W, H);
   }
