@@ -61,6 +61,33 @@ def kernel_bytes(name, scene, covered, texW, texH):
     return None
 
 
+def atomic_peak(dev, reps=3):
+    """Float-atomic throughput of this GPU (atomics/s) from the library's calibration kernel:
+    scattered (hashed) and streaming (consecutive lanes, consecutive addresses) patterns over a
+    12 MiB buffer, best of `reps`."""
+    import ctypes
+    import torch
+    from gvv_amd import _lib
+    L = _lib.load()
+    n, updates = 3 << 20, 1 << 26
+    buf = torch.zeros(n, device=dev)
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    out = {}
+    for name, scattered in (("scattered", 1), ("streaming", 0)):
+        best = None
+        for _ in range(reps + 1):
+            a, z = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            _lib.check(L.gvv_calibrate_atomics(ctypes.c_void_p(buf.data_ptr()), n, updates, scattered, stream),
+                       "gvv_calibrate_atomics")
+            z.record()
+            torch.cuda.synchronize()
+            ms = a.elapsed_time(z)
+            best = ms if best is None else min(best, ms)
+        out[name] = updates / (best / 1e3)
+    return out
+
+
 def cpu_baseline(scene, threads, views=8, min_seconds=10.0, max_reps=20):
     """Oracle forward + backward on `views` of the views, multithreaded, repeated until at least
     min_seconds of CPU work has been timed (bounded sample); returns the mean rate."""
@@ -256,6 +283,20 @@ def main():
                         frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, algorithmic_bytes=dom_bytes,
                         avg_ms=round(dom_ms, 4))
 
+    atomics = None
+    af = os.path.join(ROOT, "profiles", "atomics.json")
+    if os.path.exists(af):
+        with open(af) as fh:
+            counts = json.load(fh)
+        peak = atomic_peak(dev)
+        atomics = {}
+        for k in ("k_pixel_grad", "k_raster"):
+            if k in counts and k in kernels:
+                rate = counts[k] / (kernels[k][0] / kernels[k][1] / 1e3)
+                atomics[k] = dict(count=counts[k], rate_per_s=round(rate, 1), peak_per_s=round(peak["streaming"], 1),
+                                  frac=round(rate / peak["streaming"], 4))
+        atomics["peak_scattered_per_s"] = round(peak["scattered"], 1)
+
     if args.calibrate_copy > 0:
         n = args.calibrate_copy * (1 << 20) // 4
         src = torch.ones(n, device=dev)
@@ -296,6 +337,7 @@ def main():
                    "texture": [texH, texW], "modes": "textured+shaded", "covered_px": covered,
                    "triangles_per_s": round(scene.B * scene.C * scene.F / (ms / 1e3), 1)},
         "roofline": roofline,
+        "atomics": atomics,
         "cpu_baseline": cpu,
         "kernels_ms": {k: round(v[0] / v[1], 4) for k, v in kernels.items()},
         "fit": fit,

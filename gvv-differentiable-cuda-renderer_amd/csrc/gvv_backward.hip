@@ -128,7 +128,7 @@ __device__ __forceinline__ void pixel_terms(const BackwardArgs& a, int bc, int b
     const float ba = a.bary[2 * idx], bb = a.bary[2 * idx + 1];
     abc = v3(ba, bb, 1.f - ba - bb);
   } else {
-    abc = ray_barycentric(tri_setup(p0, p1, p2), o / 1000.f, d);
+    abc = ray_barycentric(a.tris[(size_t)b * a.F + idf], cam_origin_scaled(cam), d);
   }
   const float w3[3] = {abc.x, abc.y, abc.z};
   const float* vnb = a.vnormal + (size_t)bc * a.N * 3;
@@ -261,33 +261,24 @@ __device__ __forceinline__ void pixel_terms(const BackwardArgs& a, int bc, int b
 
 // One wavefront per TILE x TILE pixel tile of one (b, c) image (TILE = 8, 16 or 32), in
 // (TILE/8)^2 rounds of 8x8 pixels.
-// Per-vertex terms: each covered pixel's three corners are inserted into an LDS hash table
-// keyed by vertex id (lane-parallel compare-and-swap, linear probing); lanes add their
-// position / vertex-normal / colour terms into the corner's slot with LDS atomics, and the
-// table is flushed with lane-parallel global atomics (one per vertex and component) when it
-// fills past a round's worth of headroom and at the end of the tile.  A vertex is shared by
-// about six faces, so this issues about a third of the global atomics of a per-face table.
+// Per-face terms: every distinct face of the tile gets a slot in LDS (found by a ballot over
+// the slot table); lanes add their terms into their face's slot with LDS atomics, and the
+// slots are flushed with lane-parallel global atomics (one per face, vertex and component).
 // SH terms accumulate in registers over the tile and are reduced once into the tile's
 // partial slots.
-constexpr int kVertTable = 256;                // power of two; a round inserts at most 192 keys
-
-__device__ __forceinline__ int vert_hash(int v) { return (int)(((unsigned)v * 2654435761u) >> 24); }
+constexpr int kFaceSlots = 96;
 
 template <int ALBEDO, bool SHADED, bool M2D, bool HAVE_BARY, int TILE>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_pixel_grad(BackwardArgs a) {
   static_assert(TILE == 8 || TILE == 16 || TILE == 32, "tile edge must be 8, 16 or 32");
-  static_assert(kVertTable == 256, "vert_hash yields 8 bits");
   constexpr int RT = TILE / 8;                            // 8x8 rounds per tile edge
   constexpr bool NEED_POS = SHADED || M2D;
   constexpr bool NEED_COL = ALBEDO == kVertexColor;
   constexpr int G_POS = 0, G_VN = NEED_POS ? 9 : 0, G_COL = G_VN + (SHADED ? 9 : 0);
-  constexpr int NT = G_COL + (NEED_COL ? 9 : 0);         // per-face terms (3 corners x groups x 3)
+  constexpr int NT = G_COL + (NEED_COL ? 9 : 0);         // per-face terms
   constexpr int NT1 = NT > 0 ? NT : 1;
-  constexpr int NV = NT / 3;                              // per-vertex terms (groups x 3)
-  constexpr int NV1 = NV > 0 ? NV : 1;
-  __shared__ float s_vacc[kVertTable][NV1];
-  __shared__ int s_vkey[kVertTable];
-  __shared__ int s_vcount;
+  __shared__ float s_acc[kFaceSlots][NT1];
+  __shared__ int s_face[kFaceSlots];
   const int lane = threadIdx.x;
   const int tiles = a.tiles_x * a.tiles_y;
   const int64_t work = xcd_contiguous(blockIdx.x, gridDim.x);    // one (view, tile) per workgroup
@@ -298,33 +289,23 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
 #pragma unroll
   for (int k = 0; k < 27; ++k) sh_acc[k] = 0.f;
   bool any = false;
-  if (NT > 0) {
-    for (int e = lane; e < kVertTable; e += 64) s_vkey[e] = -1;
-    if (lane == 0) s_vcount = 0;
-    __syncthreads();
-  }
+  int nslots = 0;
 
-  // group g of per-vertex term k = g * 3 + component
-  auto group_base = [&](int g) -> float* {
-    return g * 9 == G_POS && NEED_POS ? a.g_pos : (g * 9 == G_VN && SHADED ? a.g_vn : a.g_col);
-  };
   auto flush = [&]() {
     __syncthreads();
-    for (int e = lane; e < kVertTable; e += 64) {
-      const int v = s_vkey[e];
-      if (v < 0) continue;
-#pragma unroll
-      for (int k = 0; k < NV; ++k) {
-        const float val = s_vacc[e][k];
-        if (val != 0.f) atomicAdd(group_base(k / 3) + ((size_t)b * a.N + v) * 3 + k % 3, val);
-      }
-      s_vkey[e] = -1;
+    for (int i = lane; i < nslots * NT; i += 64) {
+      const int slot = i / NT, k = i % NT;
+      const float v = s_acc[slot][k];
+      if (v == 0.f) continue;
+      const int grp = k / 9, kk = k % 9;
+      const int vtx = a.faces[3 * s_face[slot] + kk / 3];
+      float* base = grp * 9 == G_POS && NEED_POS ? a.g_pos : (grp * 9 == G_VN && SHADED ? a.g_vn : a.g_col);
+      atomicAdd(base + ((size_t)b * a.N + vtx) * 3 + kk % 3, v);
     }
-    if (lane == 0) s_vcount = 0;
     __syncthreads();
+    nslots = 0;
   };
 
-  bool inserted = false;
   for (int round = 0; round < RT * RT; ++round) {
     const int px = tx * TILE + (round % RT) * 8 + (lane & 7);
     const int py = ty * TILE + (round / RT) * 8 + (lane >> 3);
@@ -332,7 +313,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
     const size_t idx = ((size_t)bc * a.H + (valid ? py : 0)) * a.W + (valid ? px : 0);
     int idf = valid ? a.face[idx] : -1;
     if (idf >= a.F) idf = -1;   // ids outside the mesh are treated as background
-    if (!__ballot(idf >= 0)) continue;
+    uint64_t pending = __ballot(idf >= 0);
+    if (!pending) continue;
     any = true;
     float t[NT1 + 9];     // +9: unused groups of pixel_terms write past NT1 only in dead code
 #pragma unroll
@@ -340,44 +322,37 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
     if (idf >= 0)
       pixel_terms<ALBEDO, SHADED, M2D, HAVE_BARY, G_POS, G_VN, G_COL>(a, bc, b, px, py, idx, idf, t, sh_acc);
     if (NT == 0) continue;
-    // room for this round's at most 3 x 64 new keys (s_vcount is settled: the last writes
-    // to it were followed by a barrier)
-    if (s_vcount > kVertTable - 192) flush();
-    int slot[3] = {-1, -1, -1};
-    if (idf >= 0) {
-#pragma unroll
-      for (int r = 0; r < 3; ++r) {
-        const int v = a.faces[3 * idf + r];
-        int h = vert_hash(v);
-        for (int probe = 0; probe < kVertTable; ++probe) {
-          const int old = atomicCAS(&s_vkey[h], -1, v);
-          if (old == -1) {
-#pragma unroll
-            for (int k = 0; k < NV; ++k) s_vacc[h][k] = 0.f;
-            atomicAdd(&s_vcount, 1);
-            slot[r] = h;
-            break;
-          }
-          if (old == v) { slot[r] = h; break; }
-          h = (h + 1) & (kVertTable - 1);
-        }
+    // assign each distinct face of this round a slot (reusing slots of earlier rounds); a
+    // round adds at most 64 faces, so flushing first whenever fewer remain keeps every slot
+    // of this round valid until its terms are added
+    if (nslots > kFaceSlots - 64) flush();
+    int myslot = -1;
+    while (pending) {
+      const int leader = __ffsll((unsigned long long)pending) - 1;
+      const int f = __builtin_amdgcn_readlane(idf, leader);
+      const bool mine = idf == f;
+      pending &= ~__ballot(mine);
+      int slot = -1;
+      for (int base = 0; base < nslots && slot < 0; base += 64) {
+        const int j = base + lane;
+        const uint64_t hit = __ballot(j < nslots && s_face[j] == f);
+        if (hit) slot = base + __ffsll((unsigned long long)hit) - 1;
       }
+      if (slot < 0) {
+        slot = nslots++;
+        if (lane == 0) s_face[slot] = f;
+        for (int k = lane; k < NT; k += 64) s_acc[slot][k] = 0.f;
+      }
+      if (mine) myslot = slot;
     }
-    inserted = true;
-    __syncthreads();   // every new slot is keyed and zeroed before any term is added
+    __syncthreads();
+    if (myslot >= 0) {
 #pragma unroll
-    for (int r = 0; r < 3; ++r) {
-      if (slot[r] < 0) continue;
-#pragma unroll
-      for (int g = 0; g < NV / 3; ++g)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const float val = t[g * 9 + 3 * r + c];
-          if (val != 0.f) atomicAdd(&s_vacc[slot[r]][g * 3 + c], val);
-        }
+      for (int k = 0; k < NT; ++k)
+        if (t[k] != 0.f) atomicAdd(&s_acc[myslot][k], t[k]);
     }
   }
-  if (NT > 0 && inserted) flush();
+  if (NT > 0 && nslots > 0) flush();
 
   if (SHADED) {
 #pragma unroll
@@ -453,6 +428,11 @@ hipError_t launch_backward(const BackwardPlan& p, hipStream_t s) {
   {
     ProfScope ps(p.prof, "k_cameras", s);
     hipError_t e = launch_cameras(p.extr, p.intr, p.cams, a.B * a.C, s);
+    if (e != hipSuccess) return e;
+  }
+  if (!a.bary) {
+    ProfScope ps(p.prof, "k_face_setup", s);
+    hipError_t e = launch_face_setup(a.vpos, a.faces, nullptr, a.tris, a.B, a.N, a.F, s);
     if (e != hipSuccess) return e;
   }
   {

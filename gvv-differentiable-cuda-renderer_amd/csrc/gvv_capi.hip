@@ -113,6 +113,7 @@ int plan_forward(Handle* h, int B, int texH, int texW, const float* vpos, const 
   size_t old_keys = h->keys.n;
   GVV_CHECK(h->cams.reserve((size_t)B * h->C));
   GVV_CHECK(h->fnormal.reserve((size_t)B * h->F * 3));
+  GVV_CHECK(h->tris.reserve((size_t)B * h->F));
   GVV_CHECK(h->proj.reserve((size_t)B * h->C * h->N));
   GVV_CHECK(h->keys.reserve((size_t)pix));
   GVV_CHECK(h->counters.reserve(2));
@@ -134,14 +135,14 @@ int plan_forward(Handle* h, int B, int texH, int texW, const float* vpos, const 
   p.vpos = vpos; p.extr = extr; p.intr = intr;
   p.faces = h->d_faces.p; p.csr_start = h->d_csr_start.p; p.csr_count = h->d_csr_count.p; p.csr_list = h->d_csr_list.p;
   p.uv_table = h->d_uv_table.p;
-  p.cams = h->cams.p; p.fnormal = h->fnormal.p; p.proj = h->proj.p;
+  p.cams = h->cams.p; p.fnormal = h->fnormal.p; p.tris = h->tris.p; p.proj = h->proj.p;
   p.counters = h->counters.p; p.large = h->large.p; p.large_box = h->large_box;
   p.vnormal_out = vnormal;
   p.normal_map_out = normal_map;
   ResolveArgs& r = p.r;
-  r.B = B; r.C = h->C; r.N = h->N; r.W = h->W; r.H = h->H; r.texW = texW; r.texH = texH;
+  r.B = B; r.C = h->C; r.N = h->N; r.F = h->F; r.W = h->W; r.H = h->H; r.texW = texW; r.texH = texH;
   r.keys = h->keys.p; r.vpos = vpos; r.vcol = vcol; r.tex = tex; r.sh = sh; r.uv = h->d_uv.p;
-  r.faces = h->d_faces.p; r.cams = h->cams.p; r.vnormal = vnormal;
+  r.faces = h->d_faces.p; r.tris = h->tris.p; r.cams = h->cams.p; r.vnormal = vnormal;
   r.face = face; r.bary = bary; r.render = render;
   p.prof = nullptr;
   return 0;
@@ -162,6 +163,7 @@ int plan_backward(Handle* h, int B, int texH, int texW, const float* g_render, c
   const int tx = tiles(h->W, h->tile), ty = tiles(h->H, h->tile);
   GVV_CHECK(h->cams.reserve((size_t)B * h->C));
   GVV_CHECK(h->g_vn.reserve((size_t)nv));
+  GVV_CHECK(h->tris.reserve((size_t)B * h->F));
   GVV_CHECK(h->sh_partial.reserve((size_t)B * h->C * tx * ty * 27));
   p.albedo = h->albedo;
   p.shaded = h->shading == kShaded;
@@ -172,7 +174,7 @@ int plan_backward(Handle* h, int B, int texH, int texW, const float* g_render, c
   a.image_fs = h->image_fs; a.tile = h->tile; a.tiles_x = tx; a.tiles_y = ty;
   a.g_render = g_render; a.g_target = g_target; a.target = target;
   a.vpos = vpos; a.vcol = vcol; a.tex = tex; a.sh = sh; a.vnormal = vnormal; a.bary = bary; a.face = face;
-  a.faces = h->d_faces.p; a.uv = h->d_uv.p; a.cams = h->cams.p;
+  a.faces = h->d_faces.p; a.uv = h->d_uv.p; a.tris = h->tris.p; a.cams = h->cams.p;
   a.g_pos = g_pos; a.g_col = g_col; a.g_tex = g_tex; a.g_vn = h->g_vn.p; a.sh_partial = h->sh_partial.p; a.g_sh = g_sh;
   return 0;
 }

@@ -2,7 +2,8 @@
 //
 // Replaces renderBuffersGPU (reference cpp/src/Renderer/CUDABasedRasterization.cu:449-473):
 //   k_cameras       per-view inverse matrices and ray origin           (.cu:23-67)
-//   k_face_normals  unnormalised world-space face normals, once per mesh (.cu:122-141)
+//   k_face_setup    unnormalised world-space face normals and the scaled ray/triangle
+//                   constants, once per mesh and face rather than per view (.cu:122-141)
 //   k_vertices      vertex normals by CSR gather + per-view projection  (.cu:98-115,148-174)
 //   k_raster        per (view, face) bounding-box walk; 64-bit (depth key | face id)
 //                   atomicMin into the visibility buffer               (.cu:184-255)
@@ -23,16 +24,22 @@ __global__ void __launch_bounds__(256) k_cameras(const float* __restrict__ extr,
   cams[i] = c;
 }
 
-__global__ void __launch_bounds__(256) k_face_normals(const float* __restrict__ vpos, const int32_t* __restrict__ faces,
-                                                      float* __restrict__ fnormal, int B, int N, int F) {
+// One thread per (b, face): the unnormalised world-space face normal (for the vertex normals)
+// and the scaled ray/triangle constants shared by every view and pixel of the face.
+__global__ void __launch_bounds__(256) k_face_setup(const float* __restrict__ vpos, const int32_t* __restrict__ faces,
+                                                    float* __restrict__ fnormal, TriSetup* __restrict__ tris,
+                                                    int B, int N, int F) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (int64_t)B * F) return;
   int b = (int)(i / F), f = (int)(i % F);
   const float* vp = vpos + (size_t)b * N * 3;
   V3 v0 = ld3(vp + 3 * faces[3 * f]), v1 = ld3(vp + 3 * faces[3 * f + 1]), v2 = ld3(vp + 3 * faces[3 * f + 2]);
-  V3 n = cross(v1 - v0, v2 - v0);
-  float* o = fnormal + 3 * i;
-  o[0] = n.x; o[1] = n.y; o[2] = n.z;
+  if (fnormal) {
+    V3 n = cross(v1 - v0, v2 - v0);
+    float* o = fnormal + 3 * i;
+    o[0] = n.x; o[1] = n.y; o[2] = n.z;
+  }
+  tris[i] = tri_setup(v0, v1, v2);
 }
 
 // One thread per (b, c, vertex): vertex normal as the CSR-ordered sum of incident face
@@ -106,7 +113,7 @@ __device__ __forceinline__ void flush_candidates(const FaceStage* st, const unsi
     const int u = fs.x0 + (p - dy * fs.bw), v = fs.y0 + dy;
     const Camera& cam = cams[one_view ? bc0 : fs.bc];
     const V3 d = pixel_ray(cam, u + 0.5f, v + 0.5f);
-    const V3 abc = ray_barycentric(fs.t, cam_origin(cam) / 1000.f, d);
+    const V3 abc = ray_barycentric(fs.t, cam_origin_scaled(cam), d);
     if (!inside(abc)) continue;
     const uint64_t k = pack_key(depth_key(abc, fs.z0, fs.z1, fs.z2), fs.f);
     atomicMin(keys + ((size_t)fs.bc * H + v) * W + u, (unsigned long long)k);
@@ -161,7 +168,7 @@ constexpr int kTinyBox = 16;
 // One wavefront per 64 consecutive (b, c, face) items.  Faces with bounding boxes up to
 // large_box pixels are staged and rasterised here (pre-test, compaction, exact test); larger
 // ones are appended to `large` for k_raster_large.
-__global__ void __launch_bounds__(64) k_raster(const float* __restrict__ vpos, const int32_t* __restrict__ faces,
+__global__ void __launch_bounds__(64) k_raster(const TriSetup* __restrict__ tris, const int32_t* __restrict__ faces,
                                                const float4* __restrict__ proj, const Camera* __restrict__ cams,
                                                unsigned long long* __restrict__ keys, unsigned int* __restrict__ large_count,
                                                int2* __restrict__ large, int B, int C, int N, int F, int W, int H,
@@ -187,9 +194,8 @@ __global__ void __launch_bounds__(64) k_raster(const float* __restrict__ vpos, c
         large[slot] = make_int2(bc, f);
       } else {
         small = true;
-        const float* vp = vpos + (size_t)(bc / C) * N * 3;
         FaceStage s;
-        s.t = tri_setup(ld3(vp + 3 * r.i0), ld3(vp + 3 * r.i1), ld3(vp + 3 * r.i2));
+        s.t = tris[(size_t)(bc / C) * F + f];
         s.z0 = r.p0.z; s.z1 = r.p1.z; s.z2 = r.p2.z;
         s.f = f; s.x0 = r.bx.x0; s.y0 = r.bx.y0; s.bw = bw; s.bc = bc;
         st[lane] = s;
@@ -244,11 +250,11 @@ __global__ void __launch_bounds__(64) k_raster(const float* __restrict__ vpos, c
 // conservative span of each row (row_span; a wave prefix sum turns spans into pixel indices),
 // applies the edge pre-test, compacts and runs the exact test as k_raster does.  Every
 // wavefront leaves when the cursor passes the queue length written by k_raster.
-__global__ void __launch_bounds__(64) k_raster_large(const float* __restrict__ vpos, const int32_t* __restrict__ faces,
+__global__ void __launch_bounds__(64) k_raster_large(const TriSetup* __restrict__ tris, const int32_t* __restrict__ faces,
                                                      const float4* __restrict__ proj, const Camera* __restrict__ cams,
                                                      unsigned long long* __restrict__ keys,
                                                      unsigned int* __restrict__ counters, const int2* __restrict__ large,
-                                                     int C, int N, int W, int H) {
+                                                     int C, int N, int F, int W, int H) {
   __shared__ FaceStage st[1];
   __shared__ unsigned cand[kCandCap];
   __shared__ int s_xl[64], s_start[64];
@@ -260,12 +266,11 @@ __global__ void __launch_bounds__(64) k_raster_large(const float* __restrict__ v
     if (item >= n_large) break;
     const int bc = large[item].x, f = large[item].y;
     const RasterFace r = load_face(faces, proj + (size_t)bc * N, f, W, H);
-    const float* vp = vpos + (size_t)(bc / C) * N * 3;
     const int bw = r.bx.x1 - r.bx.x0 + 1, bh = r.bx.y1 - r.bx.y0 + 1;
     __syncthreads();
     if (threadIdx.x == 0) {
       FaceStage s;
-      s.t = tri_setup(ld3(vp + 3 * r.i0), ld3(vp + 3 * r.i1), ld3(vp + 3 * r.i2));
+      s.t = tris[(size_t)(bc / C) * F + f];
       s.z0 = r.p0.z; s.z1 = r.p1.z; s.z2 = r.p2.z;
       s.f = f; s.x0 = r.bx.x0; s.y0 = r.bx.y0; s.bw = bw; s.bc = bc;
       st[0] = s;
@@ -349,11 +354,9 @@ __global__ void __launch_bounds__(256) k_resolve(ResolveArgs a) {
   int py = pix / a.W, px = pix % a.W;
   int f = (int)(uint32_t)(key & 0xffffffffu);
   int i0 = a.faces[3 * f], i1 = a.faces[3 * f + 1], i2 = a.faces[3 * f + 2];
-  const float* vp = a.vpos + (size_t)b * a.N * 3;
-  TriSetup t = tri_setup(ld3(vp + 3 * i0), ld3(vp + 3 * i1), ld3(vp + 3 * i2));
   const Camera& cam = a.cams[bc];
   V3 d = pixel_ray(cam, px + 0.5f, py + 0.5f);
-  V3 abc = ray_barycentric(t, cam_origin(cam) / 1000.f, d);
+  V3 abc = ray_barycentric(a.tris[(size_t)b * a.F + f], cam_origin_scaled(cam), d);
   a.face[i] = f;
   a.bary[2 * i] = abc.x; a.bary[2 * i + 1] = abc.y;
 
@@ -425,6 +428,12 @@ hipError_t launch_fill_f32(float* p, int64_t n, float v, hipStream_t s) {
   return hipGetLastError();
 }
 
+hipError_t launch_face_setup(const float* vpos, const int32_t* faces, float* fnormal, TriSetup* tris, int B, int N,
+                             int F, hipStream_t s) {
+  hipLaunchKernelGGL(k_face_setup, dim3(blocks_for((int64_t)B * F)), dim3(256), 0, s, vpos, faces, fnormal, tris, B, N, F);
+  return hipGetLastError();
+}
+
 hipError_t launch_cameras(const float* extr, const float* intr, Camera* cams, int n, hipStream_t s) {
   hipLaunchKernelGGL(k_cameras, dim3(blocks_for(n)), dim3(256), 0, s, extr, intr, cams, n);
   return hipGetLastError();
@@ -444,9 +453,9 @@ hipError_t launch_forward(const ForwardPlan& p, hipStream_t s) {
     if (e != hipSuccess) return e;
   }
   {
-    ProfScope ps(p.prof, "k_face_normals", s);
-    hipLaunchKernelGGL(k_face_normals, dim3(blocks_for((int64_t)B * F)), dim3(256), 0, s, p.vpos, p.faces,
-                       p.fnormal, B, N, F);
+    ProfScope ps(p.prof, "k_face_setup", s);
+    hipError_t e = launch_face_setup(p.vpos, p.faces, p.fnormal, p.tris, B, N, F, s);
+    if (e != hipSuccess) return e;
   }
   {
     ProfScope ps(p.prof, "k_vertices", s);
@@ -462,12 +471,12 @@ hipError_t launch_forward(const ForwardPlan& p, hipStream_t s) {
     {
       ProfScope ps(p.prof, "k_raster", s);
       hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, p.counters, 2);
-      hipLaunchKernelGGL(k_raster, dim3(blocks_for((int64_t)B * C * F, 64)), dim3(64), 0, s, p.vpos, p.faces, p.proj,
+      hipLaunchKernelGGL(k_raster, dim3(blocks_for((int64_t)B * C * F, 64)), dim3(64), 0, s, p.tris, p.faces, p.proj,
                          p.cams, p.r.keys, p.counters, p.large, B, C, N, F, p.r.W, p.r.H, p.large_box);
     }
     ProfScope ps(p.prof, "k_raster_large", s);
-    hipLaunchKernelGGL(k_raster_large, dim3(kLargeWorkgroups), dim3(64), 0, s, p.vpos, p.faces, p.proj, p.cams,
-                       p.r.keys, p.counters, p.large, C, N, p.r.W, p.r.H);
+    hipLaunchKernelGGL(k_raster_large, dim3(kLargeWorkgroups), dim3(64), 0, s, p.tris, p.faces, p.proj, p.cams,
+                       p.r.keys, p.counters, p.large, C, N, F, p.r.W, p.r.H);
   }
   // In normal-map mode the visibility buffer is empty, so this writes the background.
   ProfScope ps(p.prof, "k_resolve", s);

@@ -41,6 +41,7 @@ struct Handle {
   // scratch
   DevBuf<Camera> cams;
   DevBuf<float> fnormal;
+  DevBuf<TriSetup> tris;
   DevBuf<float4> proj;
   DevBuf<unsigned long long> keys;
   DevBuf<unsigned int> counters;
@@ -52,7 +53,7 @@ struct Handle {
 
   ~Handle() {
     d_faces.release(); d_csr_start.release(); d_csr_count.release(); d_csr_list.release();
-    d_uv.release(); d_uv_table.release(); cams.release(); fnormal.release(); proj.release();
+    d_uv.release(); d_uv_table.release(); cams.release(); fnormal.release(); tris.release(); proj.release();
     keys.release(); counters.release(); large.release(); g_vn.release(); sh_partial.release();
   }
 };

@@ -80,7 +80,7 @@ enum Shading { kShaded = 0, kShadeless = 1 };
 struct UvTexel { int32_t face; float a, b, c; };
 
 struct ResolveArgs {
-  int B, C, N, W, H, texW, texH;
+  int B, C, N, F, W, H, texW, texH;
   unsigned long long* keys;          // [B][C][H][W]; reset to empty after reading
   const float* vpos;                 // [B][N][3]
   const float* vcol;                 // [B][N][3]
@@ -88,6 +88,7 @@ struct ResolveArgs {
   const float* sh;                   // [B][C][27]
   const float* uv;                   // [F][6]
   const int32_t* faces;              // [F][3]
+  const TriSetup* tris;              // [B][F]
   const Camera* cams;                // [B][C]
   const float* vnormal;              // [B][C][N][3]
   int32_t* face;                     // [B][C][H][W]
@@ -110,6 +111,7 @@ struct ForwardPlan {
   const UvTexel* uv_table;
   Camera* cams;
   float* fnormal;                    // [B][F][3]
+  TriSetup* tris;                    // [B][F] scaled ray/triangle constants
   float4* proj;                      // [B][C][N]
   float* vnormal_out;                // [B][C][N][3]
   float* normal_map_out;             // [B][texH][texW][3]
@@ -136,6 +138,7 @@ struct BackwardArgs {
   const int32_t* face;
   const int32_t* faces;
   const float* uv;
+  TriSetup* tris;                    // [B][F] scratch, filled at the start of the backward
   const Camera* cams;
   float* g_pos;                      // [B][N][3]
   float* g_col;                      // [B][N][3]
@@ -158,6 +161,8 @@ struct BackwardPlan {
 hipError_t launch_fill_u64(unsigned long long* p, int64_t n, unsigned long long v, hipStream_t s);
 hipError_t launch_fill_f32(float* p, int64_t n, float v, hipStream_t s);
 hipError_t launch_cameras(const float* extr, const float* intr, Camera* cams, int n, hipStream_t s);
+hipError_t launch_face_setup(const float* vpos, const int32_t* faces, float* fnormal, TriSetup* tris, int B, int N,
+                             int F, hipStream_t s);
 hipError_t launch_forward(const ForwardPlan& p, hipStream_t s);
 hipError_t launch_backward(const BackwardPlan& p, hipStream_t s);
 

@@ -45,10 +45,12 @@ GVV_HD uint64_t pack_key(int32_t zq, int32_t face) {
 }
 constexpr uint64_t kEmptyKey = ~0ull;
 
-// Per-view constants: rows 0..2 of (K E)^-1, the ray origin, and the raw E and K.
+// Per-view constants: rows 0..2 of (K E)^-1, the ray origin (and the origin / 1000 that the
+// scaled ray/triangle test uses), and the raw E and K.
 struct Camera {
   float invP[12];
   float o[3];
+  float os[3];
   float E[12];
   float K[9];
 };
@@ -110,11 +112,13 @@ GVV_HD void camera_setup(const float* extr12, const float* intr9, Camera& cam) {
   for (int k = 0; k < 12; ++k) cam.invP[k] = out[1][k];
   float w = out[0][15];
   cam.o[0] = out[0][3] / w; cam.o[1] = out[0][7] / w; cam.o[2] = out[0][11] / w;
+  for (int k = 0; k < 3; ++k) cam.os[k] = cam.o[k] / 1000.f;
   for (int k = 0; k < 12; ++k) cam.E[k] = extr12[k];
   for (int k = 0; k < 9; ++k) cam.K[k] = intr9[k];
 }
 
 GVV_HD V3 cam_origin(const Camera& cam) { return v3(cam.o[0], cam.o[1], cam.o[2]); }
+GVV_HD V3 cam_origin_scaled(const Camera& cam) { return v3(cam.os[0], cam.os[1], cam.os[2]); }
 
 // Unit ray direction through pixel position (px, py), back-projected at depth 1000.
 GVV_HD V3 pixel_ray(const Camera& cam, float px, float py) {
@@ -141,12 +145,14 @@ GVV_HD V3 project(const Camera& cam, V3 p) {
   return v3(x / z, y / z, z);
 }
 
-// Triangle constants of the scaled ray/triangle test, shared by every pixel of the triangle.
-struct TriSetup {
+// Triangle constants of the scaled ray/triangle test, shared by every pixel and every view of
+// the triangle (64 bytes: four 16-byte loads).
+struct alignas(16) TriSetup {
   V3 v0, v1, v2;   // vertices / 1000
   V3 N;            // cross(v1 - v0, v2 - v0)
   float dv0N;      // dot(v0, N)
   float den;       // dot(N, N)
+  float pad[2];
 };
 
 GVV_HD TriSetup tri_setup(V3 v0, V3 v1, V3 v2) {
@@ -155,6 +161,7 @@ GVV_HD TriSetup tri_setup(V3 v0, V3 v1, V3 v2) {
   t.N = cross(t.v1 - t.v0, t.v2 - t.v0);
   t.dv0N = dot(t.v0, t.N);
   t.den = dot(t.N, t.N);
+  t.pad[0] = t.pad[1] = 0.f;
   return t;
 }
 

@@ -14,6 +14,7 @@ LIB_PATH = os.environ.get("GVV_AMD_LIB", os.path.join(_PKG, "libgvv_amd.so"))
 
 EXPORTS = ("gvv_last_error", "gvv_version", "gvv_create", "gvv_destroy", "gvv_forward", "gvv_backward",
            "gvv_profile_enable", "gvv_profile_report", "gvv_set_tuning", "gvv_get_tuning",
+           "gvv_calibrate_atomics",
            "gvv_conv_out_size", "gvv_conv_forward", "gvv_conv_backward_data", "gvv_conv_workspace_floats",
            "gvv_conv_backward_weight", "gvv_conv_prepare_weights", "gvv_pack_mosaic",
            "gvv_multi_create", "gvv_multi_destroy", "gvv_multi_run", "gvv_multi_use_graph", "gvv_multi_autotune",
@@ -50,6 +51,8 @@ def load():
     L.gvv_profile_enable.restype = ctypes.c_int
     L.gvv_profile_report.argtypes = [P, ctypes.c_char_p, i64]
     L.gvv_profile_report.restype = i64
+    L.gvv_calibrate_atomics.argtypes = [P, i64, i64, i32, P]
+    L.gvv_calibrate_atomics.restype = ctypes.c_int
     L.gvv_set_tuning.argtypes = [P, i32, i32]
     L.gvv_set_tuning.restype = ctypes.c_int
     L.gvv_get_tuning.argtypes = [P, ctypes.POINTER(i32), ctypes.POINTER(i32)]

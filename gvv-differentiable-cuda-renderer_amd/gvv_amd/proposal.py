@@ -231,25 +231,37 @@ class TrainLog:
 
 
 def train(net: ProposalNet, model: fit.ShapeModel, samples: int = 200_000, batch: int = 64, lr: float = 1e-3,
-          seed: int = 1, log_every: int = 0) -> TrainLog:
-    """Trains on `samples` renders generated on the fly from the fit priors (Adam, cosine decay,
-    MSE on normalised coefficients)."""
+          seed: int = 1, log_every: int = 0, warmup: float = 0.02, weights=(1.0, 1.0, 1.0)) -> TrainLog:
+    """Trains on `samples` renders generated on the fly from the fit priors: Adam, linear warm-up
+    over the first `warmup` fraction of the steps then cosine decay, weighted MSE on normalised
+    coefficients (`weights` for the shape, SH and pose groups)."""
     dev = model.device
-    mean, std = param_stats(model.cfg.n_basis, dev)
+    K = model.cfg.n_basis
+    mean, std = param_stats(K, dev)
+    w = torch.ones(K + 33, device=dev)
+    w[:K], w[K:K + 27], w[K + 27:] = weights[0], weights[1], weights[2]
+    w = w * ((K + 33) / w.sum())
     opt = torch.optim.Adam(net.parameters(), lr=lr)
     steps = max(1, samples // batch)
-    sched = torch.optim.lr_scheduler.LambdaLR(opt, lambda s: 0.5 * (1.0 + math.cos(math.pi * min(s, steps) / steps)))
+    n_warm = max(1, int(warmup * steps))
+
+    def factor(step):
+        if step < n_warm:
+            return (step + 1) / n_warm
+        return 0.5 * (1.0 + math.cos(math.pi * min(step - n_warm, steps - n_warm) / max(1, steps - n_warm)))
+
+    sched = torch.optim.lr_scheduler.LambdaLR(opt, factor)
     rng = np.random.Generator(np.random.PCG64(seed))
     losses = []
     t0 = time.perf_counter()
     for step in range(steps):
-        p = fit.sample_params(rng, batch, model.cfg.n_basis, dev)
+        p = fit.sample_params(rng, batch, K, dev)
         x = render_inputs(model, p)
         target = (p.flat() - mean) / std
         out = net.forward(x)
         diff = out - target
         loss = (diff * diff).mean()
-        net.backward(diff * (2.0 / diff.numel()))
+        net.backward(diff * w * (2.0 / diff.numel()))
         for prm, g in zip(net.parameters(), net.gw + net.gb):
             prm.grad = g
         opt.step()

@@ -95,6 +95,12 @@ int gvv_backward(void* handle, int32_t B, int32_t texH, int32_t texW, const floa
 int gvv_set_tuning(void* handle, int32_t tile, int32_t large_box);
 int gvv_get_tuning(void* handle, int32_t* tile, int32_t* large_box);
 
+/* Measurement support (no counterpart in the reference): enqueues 2^20 threads x
+ * ceil(updates / 2^20) float atomicAdd(+1.0) into buf[0, n_floats) -- scattered (hashed)
+ * addresses if `scattered`, else consecutive lanes on consecutive floats -- so that the caller
+ * can time the GPU's float-atomic throughput.  Overwrites buf. */
+int gvv_calibrate_atomics(float* buf, int64_t n_floats, int64_t updates, int32_t scattered, void* stream);
+
 /* Kernel timing (no counterpart in the reference, which has no timing code).  When enabled,
  * every kernel launch of this handle is bracketed by HIP events on its stream; enabling or
  * disabling clears the accumulated figures. */
