@@ -37,6 +37,7 @@ for _p in (PKG, ROOT):
         sys.path.insert(0, _p)
 
 HBM_PEAK_GBS = 8000.0
+VALU_PEAK_PER_S = 256 * 2.4e9
 
 
 def kernel_bytes(name, scene, covered, texW, texH):
@@ -282,6 +283,13 @@ def main():
         roofline = dict(bound="hbm", kernel=dom, achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                         frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, algorithmic_bytes=dom_bytes,
                         avg_ms=round(dom_ms, 4))
+        vf = os.path.join(ROOT, "profiles", "valu.json")
+        if os.path.exists(vf):
+            with open(vf) as fh:
+                insts = json.load(fh).get(dom)
+            if insts:
+                # wave64 VALU issue rate: 256 CUs x 4 SIMDs x one instruction per 4 clocks x 2.4 GHz
+                roofline["valu_issue_frac"] = round(insts / (dom_ms / 1e3 * VALU_PEAK_PER_S), 4)
 
     atomics = None
     af = os.path.join(ROOT, "profiles", "atomics.json")

@@ -128,7 +128,7 @@ __device__ __forceinline__ void pixel_terms(const BackwardArgs& a, int bc, int b
     const float ba = a.bary[2 * idx], bb = a.bary[2 * idx + 1];
     abc = v3(ba, bb, 1.f - ba - bb);
   } else {
-    abc = ray_barycentric(a.tris[(size_t)b * a.F + idf], cam_origin_scaled(cam), d);
+    abc = ray_barycentric(tri_setup(p0, p1, p2), cam_origin_scaled(cam), d);
   }
   const float w3[3] = {abc.x, abc.y, abc.z};
   const float* vnb = a.vnormal + (size_t)bc * a.N * 3;
@@ -428,11 +428,6 @@ hipError_t launch_backward(const BackwardPlan& p, hipStream_t s) {
   {
     ProfScope ps(p.prof, "k_cameras", s);
     hipError_t e = launch_cameras(p.extr, p.intr, p.cams, a.B * a.C, s);
-    if (e != hipSuccess) return e;
-  }
-  if (!a.bary) {
-    ProfScope ps(p.prof, "k_face_setup", s);
-    hipError_t e = launch_face_setup(a.vpos, a.faces, nullptr, a.tris, a.B, a.N, a.F, s);
     if (e != hipSuccess) return e;
   }
   {

@@ -140,9 +140,9 @@ int plan_forward(Handle* h, int B, int texH, int texW, const float* vpos, const 
   p.vnormal_out = vnormal;
   p.normal_map_out = normal_map;
   ResolveArgs& r = p.r;
-  r.B = B; r.C = h->C; r.N = h->N; r.F = h->F; r.W = h->W; r.H = h->H; r.texW = texW; r.texH = texH;
+  r.B = B; r.C = h->C; r.N = h->N; r.W = h->W; r.H = h->H; r.texW = texW; r.texH = texH;
   r.keys = h->keys.p; r.vpos = vpos; r.vcol = vcol; r.tex = tex; r.sh = sh; r.uv = h->d_uv.p;
-  r.faces = h->d_faces.p; r.tris = h->tris.p; r.cams = h->cams.p; r.vnormal = vnormal;
+  r.faces = h->d_faces.p; r.cams = h->cams.p; r.vnormal = vnormal;
   r.face = face; r.bary = bary; r.render = render;
   p.prof = nullptr;
   return 0;
@@ -163,7 +163,6 @@ int plan_backward(Handle* h, int B, int texH, int texW, const float* g_render, c
   const int tx = tiles(h->W, h->tile), ty = tiles(h->H, h->tile);
   GVV_CHECK(h->cams.reserve((size_t)B * h->C));
   GVV_CHECK(h->g_vn.reserve((size_t)nv));
-  GVV_CHECK(h->tris.reserve((size_t)B * h->F));
   GVV_CHECK(h->sh_partial.reserve((size_t)B * h->C * tx * ty * 27));
   p.albedo = h->albedo;
   p.shaded = h->shading == kShaded;
@@ -174,7 +173,7 @@ int plan_backward(Handle* h, int B, int texH, int texW, const float* g_render, c
   a.image_fs = h->image_fs; a.tile = h->tile; a.tiles_x = tx; a.tiles_y = ty;
   a.g_render = g_render; a.g_target = g_target; a.target = target;
   a.vpos = vpos; a.vcol = vcol; a.tex = tex; a.sh = sh; a.vnormal = vnormal; a.bary = bary; a.face = face;
-  a.faces = h->d_faces.p; a.uv = h->d_uv.p; a.tris = h->tris.p; a.cams = h->cams.p;
+  a.faces = h->d_faces.p; a.uv = h->d_uv.p; a.cams = h->cams.p;
   a.g_pos = g_pos; a.g_col = g_col; a.g_tex = g_tex; a.g_vn = h->g_vn.p; a.sh_partial = h->sh_partial.p; a.g_sh = g_sh;
   return 0;
 }

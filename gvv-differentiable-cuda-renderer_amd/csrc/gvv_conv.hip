@@ -334,9 +334,12 @@ __global__ void k_prepare_weights(const float* __restrict__ w, int Cout, int T, 
 }
 
 // Renders [B][V][h][w][3] fp32 -> NHWC8 bf16 mosaic [B][rows*h/d][cols*w/d][8]: view v at tile
-// (v / cols, v % cols), each output pixel the mean of a d x d block; channels 3..7 zero.
+// (v / cols, v % cols), each output pixel the mean of a d x d block.  features == 0: channels
+// 0..2 RGB, 3..7 zero.  features == 1: channels 0..2 RGB - 0.5, 3 the covered fraction of the
+// block (pixels other than the (0, 1, 0) background), 4..5 the tile-local x and y in [-1, 1],
+// 6..7 zero.  Empty tiles are zero.
 __global__ void k_pack_mosaic(const float* __restrict__ render, int B, int V, int h, int w, int cols, int d,
-                              uint16_t* __restrict__ x) {
+                              int features, uint16_t* __restrict__ x) {
   const int rows = (V + cols - 1) / cols;
   const int th = h / d, tw = w / d;
   const int H = rows * th, W = cols * tw;
@@ -346,22 +349,32 @@ __global__ void k_pack_mosaic(const float* __restrict__ render, int B, int V, in
   const int64_t r = i / W;
   const int Y = int(r % H), b = int(r / H);
   const int v = (Y / th) * cols + X / tw;
-  uint16_t c[3] = {0, 0, 0};
+  uint16_t c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   if (v < V) {
     float acc[3] = {0.f, 0.f, 0.f};
+    int covered = 0;
     const int y0 = (Y % th) * d, x0 = (X % tw) * d;
     for (int yy = 0; yy < d; ++yy) {
       const float* s = render + ((((int64_t(b) * V + v) * h) + y0 + yy) * w + x0) * 3;
       for (int xx = 0; xx < d; ++xx) {
-        acc[0] += s[3 * xx];
-        acc[1] += s[3 * xx + 1];
-        acc[2] += s[3 * xx + 2];
+        const float R = s[3 * xx], G = s[3 * xx + 1], Bc = s[3 * xx + 2];
+        acc[0] += R;
+        acc[1] += G;
+        acc[2] += Bc;
+        covered += !(R == 0.f && G == 1.f && Bc == 0.f);
       }
     }
     const float inv = 1.f / float(d * d);
-    c[0] = f2bf(acc[0] * inv); c[1] = f2bf(acc[1] * inv); c[2] = f2bf(acc[2] * inv);
+    const float off = features ? 0.5f : 0.f;
+    c[0] = f2bf(acc[0] * inv - off); c[1] = f2bf(acc[1] * inv - off); c[2] = f2bf(acc[2] * inv - off);
+    if (features) {
+      c[3] = f2bf(covered * inv);
+      c[4] = f2bf(2.f * ((X % tw) + 0.5f) / float(tw) - 1.f);
+      c[5] = f2bf(2.f * ((Y % th) + 0.5f) / float(th) - 1.f);
+    }
   }
-  *reinterpret_cast<uint4*>(x + i * 8) = make_uint4(c[0] | (uint32_t(c[1]) << 16), c[2], 0, 0);
+  *reinterpret_cast<uint4*>(x + i * 8) = make_uint4(c[0] | (uint32_t(c[1]) << 16), c[2] | (uint32_t(c[3]) << 16),
+                                                    c[4] | (uint32_t(c[5]) << 16), 0);
 }
 
 bool geom_ok(const ConvGeom& g, std::string& why) {
@@ -546,15 +559,16 @@ int gvv_conv_prepare_weights(int32_t Cout, int32_t taps, int32_t Cin, const floa
 }
 
 int gvv_pack_mosaic(const float* render, int32_t B, int32_t V, int32_t h, int32_t w, int32_t cols, int32_t down,
-                    void* x, void* stream) {
+                    int32_t features, void* x, void* stream) {
   if (B <= 0 || V <= 0 || h <= 0 || w <= 0 || cols <= 0 || down <= 0)
     return gvv::set_last_error("pack mosaic: sizes must be positive");
   if (h % down || w % down) return gvv::set_last_error("pack mosaic: view size must be a multiple of the downsampling");
+  if (features != 0 && features != 1) return gvv::set_last_error("pack mosaic: features must be 0 or 1");
   if (!render || !x || !aligned16(x)) return gvv::set_last_error("pack mosaic: null or misaligned pointer");
   const int rows = (V + cols - 1) / cols;
   const int64_t n = int64_t(B) * rows * (h / down) * cols * (w / down);
   hipLaunchKernelGGL(k_pack_mosaic, dim3(unsigned((n + 255) / 256)), dim3(256), 0, static_cast<hipStream_t>(stream),
-                     render, B, V, h, w, cols, down, static_cast<uint16_t*>(x));
+                     render, B, V, h, w, cols, down, features, static_cast<uint16_t*>(x));
   return hip_status("pack mosaic");
 }
 

@@ -354,9 +354,12 @@ __global__ void __launch_bounds__(256) k_resolve(ResolveArgs a) {
   int py = pix / a.W, px = pix % a.W;
   int f = (int)(uint32_t)(key & 0xffffffffu);
   int i0 = a.faces[3 * f], i1 = a.faces[3 * f + 1], i2 = a.faces[3 * f + 2];
+  // recomputed from the vertices: cheaper here than a 64-byte gather of the face's constants
+  const float* vp = a.vpos + (size_t)b * a.N * 3;
+  const TriSetup t = tri_setup(ld3(vp + 3 * i0), ld3(vp + 3 * i1), ld3(vp + 3 * i2));
   const Camera& cam = a.cams[bc];
   V3 d = pixel_ray(cam, px + 0.5f, py + 0.5f);
-  V3 abc = ray_barycentric(a.tris[(size_t)b * a.F + f], cam_origin_scaled(cam), d);
+  V3 abc = ray_barycentric(t, cam_origin_scaled(cam), d);
   a.face[i] = f;
   a.bary[2 * i] = abc.x; a.bary[2 * i + 1] = abc.y;
 

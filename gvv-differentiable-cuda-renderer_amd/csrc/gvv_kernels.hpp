@@ -80,7 +80,7 @@ enum Shading { kShaded = 0, kShadeless = 1 };
 struct UvTexel { int32_t face; float a, b, c; };
 
 struct ResolveArgs {
-  int B, C, N, F, W, H, texW, texH;
+  int B, C, N, W, H, texW, texH;
   unsigned long long* keys;          // [B][C][H][W]; reset to empty after reading
   const float* vpos;                 // [B][N][3]
   const float* vcol;                 // [B][N][3]
@@ -88,7 +88,6 @@ struct ResolveArgs {
   const float* sh;                   // [B][C][27]
   const float* uv;                   // [F][6]
   const int32_t* faces;              // [F][3]
-  const TriSetup* tris;              // [B][F]
   const Camera* cams;                // [B][C]
   const float* vnormal;              // [B][C][N][3]
   int32_t* face;                     // [B][C][H][W]
@@ -138,7 +137,6 @@ struct BackwardArgs {
   const int32_t* face;
   const int32_t* faces;
   const float* uv;
-  TriSetup* tris;                    // [B][F] scratch, filled at the start of the backward
   const Camera* cams;
   float* g_pos;                      // [B][N][3]
   float* g_col;                      // [B][N][3]

@@ -69,7 +69,7 @@ def load():
     L.gvv_conv_backward_weight.restype = ctypes.c_int
     L.gvv_conv_prepare_weights.argtypes = [i32, i32, i32, P, P, P, P]
     L.gvv_conv_prepare_weights.restype = ctypes.c_int
-    L.gvv_pack_mosaic.argtypes = [P, i32, i32, i32, i32, i32, i32, P, P]
+    L.gvv_pack_mosaic.argtypes = [P, i32, i32, i32, i32, i32, i32, i32, P, P]
     L.gvv_pack_mosaic.restype = ctypes.c_int
     L.gvv_multi_create.argtypes = [P, i32, i32, i32, ctypes.POINTER(P)]
     L.gvv_multi_create.restype = ctypes.c_int

@@ -3,7 +3,8 @@
 Architecture (all convolutions 3x3, stride 2, pad 1, ReLU; NHWC bf16 activations):
 
     input  256 x 256 x 8   mosaic of the 4 views, each box-filtered from the fit resolution to
-                           128 x 128 (RGB + 5 zero channels)
+                           128 x 128: RGB - 0.5, covered fraction, tile-local x and y, 2 zero
+                           channels (features=1; features=0 is plain RGB + 5 zero channels)
     conv1  128 x 128 x 32
     conv2   64 x  64 x 64
     conv3   32 x  32 x 128
@@ -38,6 +39,7 @@ N_OUT_PAD = 128
 LAYERS = ((8, 32), (32, 64), (64, 128), (128, 128), (128, 256), (256, 256))   # (Cin, Cout), 3x3 s2 p1
 INPUT_RES = 256
 VIEW_RES = 128
+FEATURES = 1                      # input channels: 1 = RGB - 0.5, coverage, tile x/y; 0 = RGB only
 
 
 def _ptr(t):
@@ -86,8 +88,10 @@ class ProposalNet:
     that `backward` needs; `backward` writes FP32 gradients of the FP32 master weights
     (self.gw, self.gb); `refresh` re-derives the bf16 copies after an optimizer step."""
 
-    def __init__(self, device=None, seed: int = 0):
+    def __init__(self, device=None, seed: int = 0, features: int = FEATURES):
         self.device = device or torch.device("cuda", torch.cuda.current_device())
+        self.features = int(features)
+        self.in_used = 6 if self.features else 3          # live input channels of the 8
         self.L = _lib.load()
         layers, H = [], INPUT_RES
         for cin, cout in LAYERS:
@@ -102,11 +106,11 @@ class ProposalNet:
             if i == len(layers) - 1:
                 std = math.sqrt(1.0 / l.Cin)
             else:
-                fan_in = l.ks * l.ks * (3 if i == 0 else l.Cin)
+                fan_in = l.ks * l.ks * (self.in_used if i == 0 else l.Cin)
                 std = math.sqrt(2.0 / fan_in)
             w = torch.randn((l.Cout, l.ks, l.ks, l.Cin), generator=g) * std
             if i == 0:
-                w[..., 3:] = 0.0                  # input channels 3..7 are padding
+                w[..., self.in_used:] = 0.0       # padding input channels
             if i == len(layers) - 1:
                 w[N_OUT:] = 0.0                   # padded outputs
             self.w.append(w.to(self.device).contiguous())
@@ -132,9 +136,12 @@ class ProposalNet:
 
     def state_dict(self):
         return {f"w{i}": w.detach().cpu() for i, w in enumerate(self.w)} | \
-               {f"b{i}": b.detach().cpu() for i, b in enumerate(self.b)}
+               {f"b{i}": b.detach().cpu() for i, b in enumerate(self.b)} | \
+               {"features": torch.tensor(self.features)}
 
     def load_state_dict(self, sd):
+        if "features" in sd and int(sd["features"]) != self.features:
+            raise ValueError(f"weights were trained with features={int(sd['features'])}, net has {self.features}")
         for i in range(len(self.layers)):
             self.w[i].copy_(sd[f"w{i}"].to(self.device, torch.float32))
             self.b[i].copy_(sd[f"b{i}"].to(self.device, torch.float32))
@@ -185,13 +192,12 @@ class ProposalNet:
             _lib.check(self.L.gvv_conv_backward_data(B, l.H, l.H, l.Cin, l.Cout, l.ks, l.stride, l.pad, _ptr(dz),
                                                      _ptr(self.wt[i]), _ptr(x), _ptr(dx), s), "gvv_conv_backward_data")
             dz = dx
-        if self.gw[0].shape[-1] > 3:
-            self.gw[0][..., 3:] = 0.0
+        self.gw[0][..., self.in_used:] = 0.0
         self.gw[-1][N_OUT:] = 0.0
         self.gb[-1][N_OUT:] = 0.0
 
 
-def pack_mosaic(render, cols: int | None = None):
+def pack_mosaic(render, cols: int | None = None, features: int = FEATURES):
     """[B, V, R, R, 3] float32 renders -> the network input [B, 256, 256, 8] bf16: views box-filtered
     to 128 x 128 and tiled (view v at tile (v // cols, v % cols))."""
     B, V, R = render.shape[0], render.shape[1], render.shape[2]
@@ -206,21 +212,22 @@ def pack_mosaic(render, cols: int | None = None):
         x.zero_()
         view = x[:, :rows * VIEW_RES, :cols * VIEW_RES]
         tmp = torch.empty((B, rows * VIEW_RES, cols * VIEW_RES, 8), dtype=torch.bfloat16, device=render.device)
-        _lib.check(_lib.load().gvv_pack_mosaic(_ptr(render), B, V, R, R, cols, R // VIEW_RES, _ptr(tmp), _stream()),
+        _lib.check(_lib.load().gvv_pack_mosaic(_ptr(render), B, V, R, R, cols, R // VIEW_RES, features, _ptr(tmp),
+                                               _stream()),
                    "gvv_pack_mosaic")
         view.copy_(tmp)
         return x
-    _lib.check(_lib.load().gvv_pack_mosaic(_ptr(render), B, V, R, R, cols, R // VIEW_RES, _ptr(x), _stream()),
+    _lib.check(_lib.load().gvv_pack_mosaic(_ptr(render), B, V, R, R, cols, R // VIEW_RES, features, _ptr(x), _stream()),
                "gvv_pack_mosaic")
     return x
 
 
-def render_inputs(model: fit.ShapeModel, p: fit.Params):
+def render_inputs(model: fit.ShapeModel, p: fit.Params, features: int = FEATURES):
     """Network input for parameters p: the fit workload's own render (all views at the fit
     resolution), packed by pack_mosaic -- the same path the fit's target images take."""
     with torch.no_grad():
         render = model.render(p)[0]
-    return pack_mosaic(render)
+    return pack_mosaic(render, features=features)
 
 
 @dataclass
@@ -256,7 +263,7 @@ def train(net: ProposalNet, model: fit.ShapeModel, samples: int = 200_000, batch
     t0 = time.perf_counter()
     for step in range(steps):
         p = fit.sample_params(rng, batch, K, dev)
-        x = render_inputs(model, p)
+        x = render_inputs(model, p, net.features)
         target = (p.flat() - mean) / std
         out = net.forward(x)
         diff = out - target
@@ -277,7 +284,7 @@ def train(net: ProposalNet, model: fit.ShapeModel, samples: int = 200_000, batch
 def propose(net: ProposalNet, model: fit.ShapeModel, target_render) -> fit.Params:
     """Parameters predicted from target images [B, V, R, R, 3]."""
     mean, std = param_stats(model.cfg.n_basis, model.device)
-    out = net.forward(pack_mosaic(target_render))
+    out = net.forward(pack_mosaic(target_render, features=net.features))
     return fit.Params.from_flat(mean + std * out, model.cfg.n_basis)
 
 

@@ -52,10 +52,12 @@ int gvv_conv_prepare_weights(int32_t Cout, int32_t taps, int32_t Cin, const floa
                              void* stream);
 
 /* fp32 renders [B][V][h][w][3] -> bf16 NHWC8 mosaic [B][ceil(V/cols)*h/down][cols*w/down][8]:
- * view v at tile (v / cols, v % cols), each pixel the mean of a down x down block of the view;
- * channels 3..7 and empty tiles are zero.  h and w must be multiples of down. */
+ * view v at tile (v / cols, v % cols), each pixel the mean of a down x down block of the view.
+ * features 0: channels 0..2 RGB, 3..7 zero.  features 1: channels 0..2 RGB - 0.5, 3 the covered
+ * fraction of the block (pixels other than the (0, 1, 0) background), 4..5 the tile-local x, y
+ * in [-1, 1], 6..7 zero.  Empty tiles are zero.  h and w must be multiples of down. */
 int gvv_pack_mosaic(const float* render, int32_t B, int32_t V, int32_t h, int32_t w, int32_t cols, int32_t down,
-                    void* x, void* stream);
+                    int32_t features, void* x, void* stream);
 
 #ifdef __cplusplus
 }

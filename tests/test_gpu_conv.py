@@ -116,20 +116,33 @@ def test_conv_forward_backward_vs_oracle(device, geom):
     check_close("backward bias", db.cpu().numpy(), rdb, bdb, False)
 
 
-def test_pack_mosaic(device):
+@pytest.mark.parametrize("features", [0, 1])
+def test_pack_mosaic(device, features):
     L = _lib.load()
     rng = np.random.default_rng(0)
     B, V, R, d = 2, 3, 8, 2
     render = rng.random((B, V, R, R, 3)).astype(np.float32)
+    render[0, 1, :3, :5] = (0.0, 1.0, 0.0)          # background pixels
     rd = torch.from_numpy(render).to(device)
     cols = 2
     x = torch.full((B, 2 * R // d, 2 * R // d, 8), 5.0, dtype=torch.bfloat16, device=device)
-    _lib.check(L.gvv_pack_mosaic(P(rd), B, V, R, R, cols, d, P(x), S()), "pack")
+    _lib.check(L.gvv_pack_mosaic(P(rd), B, V, R, R, cols, d, features, P(x), S()), "pack")
     got = x.float().cpu().numpy()
     want = np.zeros((B, 2 * R // d, 2 * R // d, 8), np.float32)
     r = R // d
+    coord = (2.0 * (np.arange(r) + 0.5) / r - 1.0).astype(np.float32)
     for v in range(V):
         ty, tx = divmod(v, cols)
-        pooled = render[:, v].reshape(B, r, d, r, d, 3).mean((2, 4))
-        want[:, ty * r:(ty + 1) * r, tx * r:(tx + 1) * r, :3] = CO.bf16_round(pooled)
-    np.testing.assert_array_equal(got, want)
+        blocks = render[:, v].reshape(B, r, d, r, d, 3)
+        pooled = blocks.mean((2, 4)).astype(np.float32)
+        tile = want[:, ty * r:(ty + 1) * r, tx * r:(tx + 1) * r]
+        if features:
+            tile[..., :3] = CO.bf16_round(pooled - 0.5)
+            bg = np.all(blocks == np.array([0.0, 1.0, 0.0], np.float32), axis=-1)
+            tile[..., 3] = CO.bf16_round((~bg).mean((2, 4)).astype(np.float32))
+            tile[..., 4] = CO.bf16_round(np.broadcast_to(coord[None, None, :], (B, r, r)))
+            tile[..., 5] = CO.bf16_round(np.broadcast_to(coord[None, :, None], (B, r, r)))
+        else:
+            tile[..., :3] = CO.bf16_round(pooled)
+    # features: one bf16 ulp of slack for the order of the block sum (values in [-0.5, 1])
+    np.testing.assert_allclose(got, want, rtol=0, atol=2.0 ** -8 if features else 0)
