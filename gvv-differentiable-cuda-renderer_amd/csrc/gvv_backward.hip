@@ -94,7 +94,7 @@ __device__ __forceinline__ void bary_position_grad(V3 of, V3 d, V3 v0f, V3 v1f, 
   const V3 Cw = w0 * C0 + w1 * C1;
   const float NCw = w0 * dot(N, C0) + w1 * dot(N, C1);
   const float NEdw = w0 * dot(N, cross(E0, d)) + w1 * dot(N, cross(E1, d));
-  const float inv_den = 1.f / den, inv_nd = 1.f / nd;
+  const float inv_den = __builtin_amdgcn_rcpf(den), inv_nd = __builtin_amdgcn_rcpf(nd);   // 1 ulp: continuous terms
   const float c2 = -2.f * NCw * inv_den * inv_den;
 #pragma unroll
   for (int v = 0; v < 3; ++v) {
@@ -116,11 +116,10 @@ __device__ __forceinline__ void bary_position_grad(V3 of, V3 d, V3 v0f, V3 v1f, 
 // (position, vertex-normal gradient and vertex-colour rows for the three corners).
 template <int ALBEDO, bool SHADED, bool M2D, bool HAVE_BARY, int G_POS, int G_VN, int G_COL>
 __device__ __forceinline__ void pixel_terms(const BackwardArgs& a, int bc, int b, int px, int py, size_t idx, int idf,
-                                            float* t, float sh_acc[27]) {
+                                            int i0, int i1, int i2, float* t, float sh_acc[27]) {
   const Camera& cam = a.cams[bc];
   const V3 o = cam_origin(cam);
   const V3 d = pixel_ray(cam, px + 0.5f, py + 0.5f);
-  const int i0 = a.faces[3 * idf], i1 = a.faces[3 * idf + 1], i2 = a.faces[3 * idf + 2];
   const float* vp = a.vpos + (size_t)b * a.N * 3;
   const V3 p0 = ld3(vp + 3 * i0), p1 = ld3(vp + 3 * i1), p2 = ld3(vp + 3 * i2);
   V3 abc;
@@ -135,7 +134,7 @@ __device__ __forceinline__ void pixel_terms(const BackwardArgs& a, int bc, int b
   const V3 n0 = ld3(vnb + 3 * i0), n1 = ld3(vnb + 3 * i1), n2 = ld3(vnb + 3 * i2);
   const V3 un = n0 * abc.x + n1 * abc.y + n2 * abc.z;
   const float unl = length(un);
-  V3 n = un / unl;
+  V3 n = un * (1.f / unl);
   const bool flipped = dot(n, d) > 0.f;
   if (flipped) n = -n;
   const float* shc = a.sh + (size_t)bc * 27;
@@ -193,7 +192,7 @@ __device__ __forceinline__ void pixel_terms(const BackwardArgs& a, int bc, int b
       gn[1] += glc[i] * (sc[1] + sc[4] * n.x + sc[5] * n.z + sc[8] * -2.f * n.y);
       gn[2] += glc[i] * (sc[2] + sc[5] * n.y + sc[6] * 6.f * n.z + sc[7] * n.x);
     }
-    const float unl2 = unl * unl, inv_unl3 = 1.f / (unl2 * unl);
+    const float unl2 = unl * unl, inv_unl3 = __builtin_amdgcn_rcpf(unl2 * unl);
     const float gnu = gn[0] * un.x + gn[1] * un.y + gn[2] * un.z;
     const V3 gu = v3((gn[0] * unl2 - un.x * gnu) * inv_unl3, (gn[1] * unl2 - un.y * gnu) * inv_unl3,
                      (gn[2] * unl2 - un.z * gnu) * inv_unl3);
@@ -306,21 +305,38 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
     nslots = 0;
   };
 
-  for (int round = 0; round < RT * RT; ++round) {
-    const int px = tx * TILE + (round % RT) * 8 + (lane & 7);
-    const int py = ty * TILE + (round / RT) * 8 + (lane >> 3);
+  // Software pipeline over the rounds: the next round's face id, and then its vertex ids, are
+  // loaded while this round computes.
+  auto round_pixel = [&](int round, int& px, int& py, size_t& idx) {
+    px = tx * TILE + (round % RT) * 8 + (lane & 7);
+    py = ty * TILE + (round / RT) * 8 + (lane >> 3);
     const bool valid = px < a.W && py < a.H;
-    const size_t idx = ((size_t)bc * a.H + (valid ? py : 0)) * a.W + (valid ? px : 0);
-    int idf = valid ? a.face[idx] : -1;
-    if (idf >= a.F) idf = -1;   // ids outside the mesh are treated as background
+    idx = ((size_t)bc * a.H + (valid ? py : 0)) * a.W + (valid ? px : 0);
+    int f = valid ? a.face[idx] : -1;
+    return f >= a.F ? -1 : f;   // ids outside the mesh are treated as background
+  };
+  int npx, npy;
+  size_t nidx;
+  int nidf = round_pixel(0, npx, npy, nidx);
+  int ni0 = 0, ni1 = 0, ni2 = 0;
+  if (nidf >= 0) { ni0 = a.faces[3 * nidf]; ni1 = a.faces[3 * nidf + 1]; ni2 = a.faces[3 * nidf + 2]; }
+  for (int round = 0; round < RT * RT; ++round) {
+    const int px = npx, py = npy, idf = nidf, i0 = ni0, i1 = ni1, i2 = ni2;
+    const size_t idx = nidx;
+    if (round + 1 < RT * RT) nidf = round_pixel(round + 1, npx, npy, nidx);
     uint64_t pending = __ballot(idf >= 0);
-    if (!pending) continue;
-    any = true;
+    if (pending) {
+      any = true;
+    }
     float t[NT1 + 9];     // +9: unused groups of pixel_terms write past NT1 only in dead code
 #pragma unroll
     for (int k = 0; k < NT1 + 9; ++k) t[k] = 0.f;
     if (idf >= 0)
-      pixel_terms<ALBEDO, SHADED, M2D, HAVE_BARY, G_POS, G_VN, G_COL>(a, bc, b, px, py, idx, idf, t, sh_acc);
+      pixel_terms<ALBEDO, SHADED, M2D, HAVE_BARY, G_POS, G_VN, G_COL>(a, bc, b, px, py, idx, idf, i0, i1, i2, t, sh_acc);
+    if (round + 1 < RT * RT && nidf >= 0) {
+      ni0 = a.faces[3 * nidf]; ni1 = a.faces[3 * nidf + 1]; ni2 = a.faces[3 * nidf + 2];
+    }
+    if (!pending) continue;
     if (NT == 0) continue;
     // assign each distinct face of this round a slot (reusing slots of earlier rounds); a
     // round adds at most 64 faces, so flushing first whenever fewer remain keeps every slot

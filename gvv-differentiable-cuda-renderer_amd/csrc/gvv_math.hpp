@@ -145,6 +145,19 @@ GVV_HD V3 project(const Camera& cam, V3 p) {
   return v3(x / z, y / z, z);
 }
 
+// x / 1000, correctly rounded, as a multiply and two FMAs (the remainder x - q*1000 of the
+// first quotient q = x * RN(1/1000) is exact; one Newton correction then rounds correctly).
+// Checked bit-for-bit against the IEEE division for every binary32 x with |x| >= 2^-120; below
+// that (subnormal quotients, signed zero) the division itself is used.
+GVV_HD float div1000(float x) {
+  const float y = 0.001f;
+  if (!(fabsf(x) >= 0x1p-120f)) return x / 1000.f;
+  const float q = x * y;
+  const float r = fmaf(-q, 1000.f, x);
+  return fmaf(r, y, q);
+}
+GVV_HD V3 div1000(V3 a) { return v3(div1000(a.x), div1000(a.y), div1000(a.z)); }
+
 // Triangle constants of the scaled ray/triangle test, shared by every pixel and every view of
 // the triangle (64 bytes: four 16-byte loads).
 struct alignas(16) TriSetup {
@@ -157,7 +170,7 @@ struct alignas(16) TriSetup {
 
 GVV_HD TriSetup tri_setup(V3 v0, V3 v1, V3 v2) {
   TriSetup t;
-  t.v0 = v0 / 1000.f; t.v1 = v1 / 1000.f; t.v2 = v2 / 1000.f;
+  t.v0 = div1000(v0); t.v1 = div1000(v1); t.v2 = div1000(v2);
   t.N = cross(t.v1 - t.v0, t.v2 - t.v0);
   t.dv0N = dot(t.v0, t.N);
   t.den = dot(t.N, t.N);

@@ -44,11 +44,12 @@ class FitConfig:
     iters: int = 200
     lr: float = 1e-2
     seed: int = 0
+    colour_smoothing: int = 0     # Laplacian steps applied to the template's random vertex colours
 
 
-def _smooth_fields(rng, faces, n, k, iters=60):
-    """k random 3-vector fields on the mesh vertices, smoothed by `iters` steps of uniform
-    Laplacian averaging over the face adjacency (smooth, low-frequency displacements)."""
+def _smooth(faces, n, X, iters):
+    """`iters` steps of uniform Laplacian averaging of per-vertex values X (n, m) over the face
+    adjacency."""
     import scipy.sparse as sp
     rows = np.concatenate([faces[:, 0], faces[:, 1], faces[:, 2], faces[:, 1], faces[:, 2], faces[:, 0]])
     cols = np.concatenate([faces[:, 1], faces[:, 2], faces[:, 0], faces[:, 0], faces[:, 1], faces[:, 2]])
@@ -56,9 +57,15 @@ def _smooth_fields(rng, faces, n, k, iters=60):
     A.data[:] = 1.0
     deg = np.asarray(A.sum(1)).ravel()
     M = sp.diags(1.0 / np.maximum(deg, 1)) @ A
-    X = rng.normal(size=(n, 3 * k))
     for _ in range(iters):
         X = 0.5 * X + 0.5 * (M @ X)
+    return X
+
+
+def _smooth_fields(rng, faces, n, k, iters=60):
+    """k random 3-vector fields on the mesh vertices, smoothed by `iters` steps of uniform
+    Laplacian averaging over the face adjacency (smooth, low-frequency displacements)."""
+    X = _smooth(faces, n, rng.normal(size=(n, 3 * k)), iters)
     return X.reshape(n, k, 3).transpose(1, 0, 2)          # (k, n, 3)
 
 
@@ -110,7 +117,12 @@ class ShapeModel:
         self.template = torch.tensor(v * MM, dtype=torch.float32, device=dev)
         self.centre = self.template.mean(0)
         self.basis = torch.tensor(Q.reshape(cfg.n_basis, -1), dtype=torch.float32, device=dev)   # (K, 3N)
-        self.colors = torch.tensor(r_col.uniform(0.2, 0.9, size=(self.N, 3)), dtype=torch.float32, device=dev)
+        col = r_col.uniform(0.2, 0.9, size=(self.N, 3))
+        if cfg.colour_smoothing > 0:
+            # low-frequency albedo: smoothed, then rescaled to the spread of U(0.2, 0.9) around 0.55
+            c = _smooth(f, self.N, col - 0.55, cfg.colour_smoothing)
+            col = np.clip(0.55 + c * (0.7 / np.sqrt(12.0) / max(c.std(), 1e-12)), 0.05, 0.95)
+        self.colors = torch.tensor(col, dtype=torch.float32, device=dev)
         E, K = scenes.ring_cameras(cfg.views, 3.5, cfg.resolution, cfg.resolution)
         self.extr = torch.tensor(E, device=dev)
         self.intr = torch.tensor(K, device=dev)
