@@ -7,8 +7,11 @@ texture and SH.  One step = one forward and one backward of the renderer over th
 every input already resident in HBM; the upstream gradient is the sum-of-squares loss gradient
 2(render - target) of the first forward, fixed across steps.
 
-Prints one JSON line: Mpix/s = 8 x 1024 x 1024 / (t_fwd + t_bwd), plus the roofline of the
-dominant kernel (HIP-event timing through the library's profiling hook), the CPU oracle timed
+Prints one JSON line: Mpix/s = 8 x 1024 x 1024 / (t_fwd + t_bwd) over K uninstrumented steps
+(launch settings chosen first by timing the autotuner's candidates), plus the roofline of the
+dominant kernel (HIP-event timing through the library's profiling hook over a second run of
+the same K steps), the step's algorithmic bytes against HBM peak, the atomic rate against a
+measured atomic peak, the CPU oracle timed
 on a bounded sample of the same workload, and (unless --no-fit) the fit time-to-tolerance of
 the proposal-warm-started fit against the plain fit:
 
@@ -37,7 +40,7 @@ for _p in (PKG, ROOT):
         sys.path.insert(0, _p)
 
 HBM_PEAK_GBS = 8000.0
-VALU_PEAK_PER_S = 256 * 2.4e9
+VALU_PEAK_PER_S = 157.3e12 / 2 / 64     # wave64 FP32 FMA instructions/s at the 157.3 TFLOP/s vector peak
 
 
 def kernel_bytes(name, scene, covered, texW, texH):
@@ -89,9 +92,20 @@ def atomic_peak(dev, reps=3):
     return out
 
 
-def cpu_baseline(scene, threads, views=8, min_seconds=10.0, max_reps=20):
-    """Oracle forward + backward on `views` of the views, multithreaded, repeated until at least
-    min_seconds of CPU work has been timed (bounded sample); returns the mean rate."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(scene, threads, views=8, reps=3):
+    """Oracle forward + backward over `views` of the views, multithreaded: one warm-up, then the
+    median of `reps` timed repetitions (one-time setup excluded)."""
     import numpy as np
     from oracle import oracle as O
     O.set_threads(threads)
@@ -102,9 +116,8 @@ def cpu_baseline(scene, threads, views=8, min_seconds=10.0, max_reps=20):
     ins = (scene.vpos, scene.vcol, scene.tex, np.ascontiguousarray(scene.sh[:, :views]),
            np.ascontiguousarray(scene.extr[:, :views]), np.ascontiguousarray(scene.intr[:, :views]))
     target = np.ascontiguousarray(scene.target[:, :views])
-    t_fwd = t_bwd = 0.0
-    reps = 0
-    while reps < max_reps and (t_fwd + t_bwd) < min_seconds:
+    times = []
+    for rep in range(reps + 1):
         t0 = time.perf_counter()
         out = orc.forward(*ins)
         t1 = time.perf_counter()
@@ -113,14 +126,26 @@ def cpu_baseline(scene, threads, views=8, min_seconds=10.0, max_reps=20):
         orc.backward(g, np.zeros_like(g), target, scene.vpos, scene.vcol, scene.tex, ins[3], out["vnormal"], out["bary"],
                      out["face"], ins[4], ins[5])
         t3 = time.perf_counter()
-        t_fwd += t1 - t0
-        t_bwd += t3 - t2
-        reps += 1
+        if rep > 0:
+            times.append((t1 - t0, t3 - t2))
+    times.sort(key=lambda t: t[0] + t[1])
+    t_fwd, t_bwd = times[len(times) // 2]
     px = views * scene.width * scene.height * scene.B
-    secs = t_fwd + t_bwd
-    return dict(value=reps * px / secs / 1e6, unit="Mpix/s", cores=threads, kind="port",
-                sample=f"FP64 oracle fwd+bwd on {views} of {scene.C} views ({px} px, {scene.F} triangles) x {reps} "
-                       f"repetitions; fwd {t_fwd / reps:.2f}s bwd {t_bwd / reps:.2f}s per repetition")
+    return dict(value=px / (t_fwd + t_bwd) / 1e6, unit="Mpix/s", cores=threads, kind="port",
+                sample=f"FP64 oracle fwd+bwd on {views} of {scene.C} views ({px} px, {scene.F} triangles), median of "
+                       f"{reps} after 1 warm-up: fwd {t_fwd:.2f}s bwd {t_bwd:.2f}s; {threads} threads on {cpu_model()}")
+
+
+def step_bytes(scene, covered, texW, texH):
+    """Algorithmic bytes of one forward + backward (every distinct input byte read once, every
+    output byte written once): per pixel 24 forward outputs + 16 backward inputs, 12 per covered
+    pixel (texel read), vertex normals out, positions, UV indices, UVs, texture read and texture
+    gradient write, position/colour gradients, SH and its gradient."""
+    B, C, N, F = scene.B, scene.C, scene.N, scene.F
+    P = B * C * scene.width * scene.height
+    T = B * texW * texH
+    return P * (24 + 16) + 12 * covered + B * C * N * 12 + B * N * 12 + 12 * F + 24 * F + 2 * 12 * T + B * N * 24 \
+        + 108 * B * C
 
 
 def fit_experiment(train_samples):
@@ -133,11 +158,12 @@ def fit_experiment(train_samples):
     cfg = fit.FitConfig()
     model = fit.ShapeModel(cfg, dev)
     net = proposal.ProposalNet(dev, seed=0)
-    log = proposal.train(net, model, samples=train_samples, batch=64, seed=1)
+    log = proposal.train(net, model, samples=train_samples, batch=32, seed=1)
     gt = fit.sample_params(np.random.Generator(np.random.PCG64(12345)), cfg.batch, cfg.n_basis, dev)
     rep = proposal.compare_fits(net, model, gt)
     return dict(metric="fit time-to-tolerance vs plain", ratio=None if rep.ratio is None else round(rep.ratio, 4),
-                plain_seconds=round(rep.plain_seconds, 4), plain_iters=rep.plain_iters,
+                plain_seconds=round(rep.plain_seconds, 4), plain_seconds_to_final=round(rep.plain_seconds_to_final, 4),
+                plain_iters=rep.plain_iters,
                 plain_final_loss=rep.plain_final_loss, plain_initial_loss=rep.plain_initial_loss,
                 warm_seconds=None if rep.warm_seconds is None else round(rep.warm_seconds, 4),
                 warm_iters=rep.warm_iters, warm_initial_loss=rep.warm_initial_loss, fallbacks=rep.fallbacks,
@@ -214,6 +240,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-views", type=int, default=8)
+    ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--kernels-json", default=None, help="also write per-kernel timings here")
     ap.add_argument("--workload", choices=("render", "multi"), default="render")
     ap.add_argument("--no-fit", action="store_true", help="skip the proposal + fit time-to-tolerance run")
@@ -252,18 +279,42 @@ def main():
         _, f, _, vn, _ = rast.forward(vpos, vcol, tex, sh, extr, intr)
         rast.backward(g_render, None, None, vpos, vcol, tex, sh, vn, f, extr, intr)
 
+    # launch settings: the fastest of the autotuner's candidates on this workload (untimed)
+    tuning_ms = {}
+    for tile in (8, 16, 32):
+        for box in (64, 256, 1024):
+            rast.set_tuning(tile, box)
+            step()
+            a, z = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            torch.cuda.synchronize()
+            a.record()
+            for _ in range(3):
+                step()
+            z.record()
+            torch.cuda.synchronize()
+            tuning_ms[(tile, box)] = a.elapsed_time(z) / 3
+    best = min(tuning_ms, key=tuning_ms.get)
+    rast.set_tuning(*best)
+
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
+    def timed(steps):
+        start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        start.record()
+        for _ in range(steps):
+            step()
+        end.record()
+        torch.cuda.synchronize()
+        return start.elapsed_time(end) / steps
+
+    # the throughput: K steps with nothing else on the stream
+    ms = timed(args.steps)
+    # the same K steps again with the library's per-kernel HIP events on its launch stream, for
+    # the per-kernel figures (the events themselves add a little to this second region)
     rast.profile(True)
-    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    start.record()
-    for _ in range(args.steps):
-        step()
-    end.record()
-    torch.cuda.synchronize()
-    ms = start.elapsed_time(end) / args.steps
+    ms_instrumented = timed(args.steps)
     kernels = rast.profile_report()
     rast.profile(False)
 
@@ -288,7 +339,7 @@ def main():
             with open(vf) as fh:
                 insts = json.load(fh).get(dom)
             if insts:
-                # wave64 VALU issue rate: 256 CUs x 4 SIMDs x one instruction per 4 clocks x 2.4 GHz
+                # share of the wave64 VALU issue rate at the FP32 vector peak
                 roofline["valu_issue_frac"] = round(insts / (dom_ms / 1e3 * VALU_PEAK_PER_S), 4)
 
     atomics = None
@@ -298,12 +349,16 @@ def main():
             counts = json.load(fh)
         peak = atomic_peak(dev)
         atomics = {}
-        for k in ("k_pixel_grad", "k_raster"):
+        for k in ("k_pixel_grad", "k_raster", "k_raster_large", "k_face_jac"):
             if k in counts and k in kernels:
                 rate = counts[k] / (kernels[k][0] / kernels[k][1] / 1e3)
-                atomics[k] = dict(count=counts[k], rate_per_s=round(rate, 1), peak_per_s=round(peak["streaming"], 1),
+                atomics[k] = dict(requests=counts[k], rate_per_s=round(rate, 1), peak_per_s=round(peak["streaming"], 1),
                                   frac=round(rate / peak["streaming"], 4))
         atomics["peak_scattered_per_s"] = round(peak["scattered"], 1)
+        # the reference formulation scatters ~210 atomics per covered pixel in textured+shaded mode
+        # (3 texel + 27 SH + 9 barycentric-position + 9 model-to-data + ~162 per-incident-face normal terms)
+        atomics["reference_formulation_per_step"] = 210 * covered
+        atomics["issued_requests_per_step"] = sum(v for k, v in counts.items() if k in kernels)
 
     if args.calibrate_copy > 0:
         n = args.calibrate_copy * (1 << 20) // 4
@@ -316,7 +371,7 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        cpu = cpu_baseline(scene, threads, views=args.cpu_views)
+        cpu = cpu_baseline(scene, threads, views=args.cpu_views, reps=args.cpu_reps)
 
     fit = None
     if not args.no_fit:
@@ -326,6 +381,10 @@ def main():
             fit = fit_experiment(args.train_samples)
         except Exception as e:      # reported in the line; the throughput figure stands on its own
             fit = dict(error=f"{type(e).__name__}: {e}")
+
+    sb = step_bytes(scene, covered, texW, texH)
+    step_roofline = dict(algorithmic_bytes=sb, achieved_GBs=round(sb / (ms / 1e3) / 1e9, 1), peak_GBs=HBM_PEAK_GBS,
+                         frac=round(sb / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4))
 
     line = {
         "metric": "fwd+bwd Mpix/s (8x1024^2 views, 100k tris)",
@@ -340,11 +399,15 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic (seeded capped cylinder, smooth-noise texture, ring cameras)",
+        "ms_per_step_with_kernel_events": round(ms_instrumented, 4),
         "config": {"workload": "cylinder100k_8x1024_textured_shaded", "triangles": scene.F,
                    "vertices": scene.N, "views": scene.C, "resolution": [scene.height, scene.width],
                    "texture": [texH, texW], "modes": "textured+shaded", "covered_px": covered,
-                   "triangles_per_s": round(scene.B * scene.C * scene.F / (ms / 1e3), 1)},
+                   "triangles_per_s": round(scene.B * scene.C * scene.F / (ms / 1e3), 1),
+                   "tuning": {"tile": best[0], "large_box": best[1],
+                              "candidates_ms": {f"{t}/{b}": round(v, 4) for (t, b), v in tuning_ms.items()}}},
         "roofline": roofline,
+        "step_roofline": step_roofline,
         "atomics": atomics,
         "cpu_baseline": cpu,
         "kernels_ms": {k: round(v[0] / v[1], 4) for k, v in kernels.items()},

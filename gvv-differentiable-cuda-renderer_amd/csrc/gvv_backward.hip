@@ -134,7 +134,7 @@ __device__ __forceinline__ void pixel_terms(const BackwardArgs& a, int bc, int b
   const V3 n0 = ld3(vnb + 3 * i0), n1 = ld3(vnb + 3 * i1), n2 = ld3(vnb + 3 * i2);
   const V3 un = n0 * abc.x + n1 * abc.y + n2 * abc.z;
   const float unl = length(un);
-  V3 n = un * (1.f / unl);
+  V3 n = un / unl;         // IEEE division: the flip decision below is part of the binary32 contract
   const bool flipped = dot(n, d) > 0.f;
   if (flipped) n = -n;
   const float* shc = a.sh + (size_t)bc * 27;

@@ -96,7 +96,7 @@ struct FaceStage {
 };
 
 // Candidate list of one wavefront: (face slot << 24 | pixel index inside the face's box).
-constexpr int kCandCap = 1536;
+constexpr int kCandCap = 512;
 
 // Exact test of every staged candidate (reference uv2barycentric + inside test + depth key,
 // CUDABasedRasterization.cu:237-250), then the 64-bit minimum into the visibility buffer.

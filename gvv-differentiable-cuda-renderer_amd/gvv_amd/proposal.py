@@ -237,7 +237,7 @@ class TrainLog:
     losses: list
 
 
-def train(net: ProposalNet, model: fit.ShapeModel, samples: int = 200_000, batch: int = 64, lr: float = 1e-3,
+def train(net: ProposalNet, model: fit.ShapeModel, samples: int = 200_000, batch: int = 32, lr: float = 1e-3,
           seed: int = 1, log_every: int = 0, warmup: float = 0.02, weights=(1.0, 1.0, 1.0)) -> TrainLog:
     """Trains on `samples` renders generated on the fly from the fit priors: Adam, linear warm-up
     over the first `warmup` fraction of the steps then cosine decay, weighted MSE on normalised
@@ -310,7 +310,8 @@ def warm_start(net: ProposalNet, model: fit.ShapeModel, target, mask):
 @dataclass
 class WarmStartReport:
     plain_final_loss: float
-    plain_seconds: float
+    plain_seconds: float                # the plain fit's wall-clock for all its iterations
+    plain_seconds_to_final: float       # until the plain loss first reached its final value
     plain_iters: int
     warm_initial_loss: float
     plain_initial_loss: float
@@ -345,6 +346,7 @@ def compare_fits(net: ProposalNet, model: fit.ShapeModel, gt: fit.Params, iters:
         if l <= level:
             it_warm = i
             break
-    return WarmStartReport(plain_final_loss=level, plain_seconds=plain.time_to(level), plain_iters=iters,
+    return WarmStartReport(plain_final_loss=level, plain_seconds=plain.times[-1],
+                           plain_seconds_to_final=plain.time_to(level), plain_iters=iters,
                            warm_initial_loss=warm.losses[0], plain_initial_loss=plain.losses[0],
                            warm_seconds=t_warm, warm_iters=it_warm, fallbacks=fallbacks)
