@@ -241,6 +241,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-views", type=int, default=8)
     ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--tuning", default=None, metavar="TILE,BOX",
+                    help="fix the backward tile edge and large-face threshold instead of choosing them")
     ap.add_argument("--kernels-json", default=None, help="also write per-kernel timings here")
     ap.add_argument("--workload", choices=("render", "multi"), default="render")
     ap.add_argument("--no-fit", action="store_true", help="skip the proposal + fit time-to-tolerance run")
@@ -279,26 +281,30 @@ def main():
         _, f, _, vn, _ = rast.forward(vpos, vcol, tex, sh, extr, intr)
         rast.backward(g_render, None, None, vpos, vcol, tex, sh, vn, f, extr, intr)
 
-    # launch settings: the fastest of the autotuner's candidates on this workload (untimed)
+    # launch settings: the fastest of the autotuner's candidates on this workload (untimed), or
+    # the ones given with --tuning
     tuning_ms = {}
-    for tile in (8, 16, 32):
-        for box in (64, 256, 1024):
-            rast.set_tuning(tile, box)
+    candidates = [(t, b) for t in (8, 16, 32) for b in (64, 256, 1024)]
+    if args.tuning:
+        candidates = [tuple(int(v) for v in args.tuning.split(","))]
+    for tile, box in candidates:
+        rast.set_tuning(tile, box)
+        step()
+        a, z = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        a.record()
+        for _ in range(3):
             step()
-            a, z = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            torch.cuda.synchronize()
-            a.record()
-            for _ in range(3):
-                step()
-            z.record()
-            torch.cuda.synchronize()
-            tuning_ms[(tile, box)] = a.elapsed_time(z) / 3
+        z.record()
+        torch.cuda.synchronize()
+        tuning_ms[(tile, box)] = a.elapsed_time(z) / 3
     best = min(tuning_ms, key=tuning_ms.get)
     rast.set_tuning(*best)
 
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
+
     def timed(steps):
         start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         torch.cuda.synchronize()
