@@ -158,7 +158,7 @@ def fit_experiment(train_samples):
     cfg = fit.FitConfig()
     model = fit.ShapeModel(cfg, dev)
     net = proposal.ProposalNet(dev, seed=0)
-    log = proposal.train(net, model, samples=train_samples, batch=32, seed=1)
+    log = proposal.train(net, model, samples=train_samples, batch=16, lr=7e-4, seed=1)
     gt = fit.sample_params(np.random.Generator(np.random.PCG64(12345)), cfg.batch, cfg.n_basis, dev)
     rep = proposal.compare_fits(net, model, gt)
     return dict(metric="fit time-to-tolerance vs plain", ratio=None if rep.ratio is None else round(rep.ratio, 4),

@@ -12,6 +12,8 @@ workload configuration it is measured on:
   pose (axis-angle rotation about the template centre, translation in units of 0.1 m).  The
   pose is applied to the vertices before rendering: the reference returns zero gradients
   for extrinsics (python/CudaRenderer.py:215), so rigid motion has to live upstream;
+* the template's vertex colours are U(0.2, 0.9) per vertex (a high-frequency albedo);
+  FitConfig.colour_smoothing > 0 smooths them over the mesh for a low-frequency albedo;
 * vertexColor + shaded rendering of 4 views at 512 x 512, masked L2 photometric loss against
   targets rendered from ground-truth parameters, Adam (lr 1e-2).
 
