@@ -293,30 +293,45 @@ __global__ __launch_bounds__(kThreads) void k_conv(ConvArgs a) {
   }
 }
 
-// out[i] = sum_s part[s][i] (deterministic split reduction)
-__global__ void k_split_sum(const float* __restrict__ part, int splits, int64_t n, float* __restrict__ out) {
-  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+// out[i] = sum_s part[s][i], deterministic.  Block: 64 consecutive outputs x 4 split lanes;
+// each lane sums every 4th split, then the four partial sums are added in fixed order.
+__global__ __launch_bounds__(256) void k_split_sum(const float* __restrict__ part, int splits, int64_t n,
+                                                   float* __restrict__ out) {
+  __shared__ float red[4][64];
+  const int64_t i = int64_t(blockIdx.x) * 64 + (threadIdx.x & 63);
+  const int r = threadIdx.x >> 6;
   float s = 0.f;
-  for (int k = 0; k < splits; ++k) s += part[int64_t(k) * n + i];
-  out[i] = s;
+  if (i < n)
+    for (int k = r; k < splits; k += 4) s += part[int64_t(k) * n + i];
+  red[r][threadIdx.x & 63] = s;
+  __syncthreads();
+  if (r == 0 && i < n) out[i] = (red[0][threadIdx.x] + red[1][threadIdx.x]) + (red[2][threadIdx.x] + red[3][threadIdx.x]);
 }
 
 // Bias gradient partials: part[split][c] = sum over the split's pixels of dz[pix][c].
-// Block: 64 channels x 4 pixel lanes; grid (ceil(C/64), splits).
+// Block: 64 channels x 4 pixel lanes; grid (ceil(C/64), splits).  Each lane keeps four
+// independent running sums so that four loads are in flight.
 __global__ __launch_bounds__(256) void k_bias_partial(const uint16_t* __restrict__ dz, int64_t pixels, int C,
                                                       int64_t per_split, float* __restrict__ part) {
   __shared__ float red[4][64];
   const int c = blockIdx.x * 64 + (threadIdx.x & 63), r = threadIdx.x >> 6;
   const int64_t beg = int64_t(blockIdx.y) * per_split;
   const int64_t end = beg + per_split < pixels ? beg + per_split : pixels;
-  float s = 0.f;
-  if (c < C)
-    for (int64_t p = beg + r; p < end; p += 4) s += bf2f(dz[p * C + c]);
-  red[r][threadIdx.x & 63] = s;
+  float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+  if (c < C) {
+    int64_t p = beg + r;
+    for (; p + 12 < end; p += 16) {
+      s0 += bf2f(dz[p * C + c]);
+      s1 += bf2f(dz[(p + 4) * C + c]);
+      s2 += bf2f(dz[(p + 8) * C + c]);
+      s3 += bf2f(dz[(p + 12) * C + c]);
+    }
+    for (; p < end; p += 4) s0 += bf2f(dz[p * C + c]);
+  }
+  red[r][threadIdx.x & 63] = (s0 + s1) + (s2 + s3);
   __syncthreads();
   if (r == 0 && c < C)
-    part[int64_t(blockIdx.y) * C + c] = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+    part[int64_t(blockIdx.y) * C + c] = (red[0][threadIdx.x] + red[1][threadIdx.x]) + (red[2][threadIdx.x] + red[3][threadIdx.x]);
 }
 
 // fp32 master weights [Cout][T][Cin] -> bf16 copy and bf16 transpose [Cin][T][Cout]
@@ -435,22 +450,23 @@ void launch_tiled(const ConvArgs& a, int splits, hipStream_t s) {
 }
 
 // Split of the weight-gradient reduction: enough (P, Q) tiles x splits to fill 256 CUs a few
-// times over, each split a multiple of kBK pixels.
+// times over, each split at least 8 k-steps (256 pixels) and at most 1024 splits; k_split_sum
+// reduces the partials in parallel over outputs and splits.
 void weight_split(const ConvGeom& g, int& splits, int64_t& per) {
   int64_t P, Q, K;
   gemm_dims(kBwdWeight, g, P, Q, K);
   const int BP = P <= 32 ? 32 : 64, BQ = P <= 32 ? 128 : (Q >= 64 * 1024 ? 128 : 64);
   const int64_t tiles = ((P + BP - 1) / BP) * ((Q + BQ - 1) / BQ);
-  int64_t want = (2048 + tiles - 1) / tiles;
   const int64_t steps = (K + kBK - 1) / kBK;
-  if (want > steps) want = steps;
+  int64_t want = (2048 + tiles - 1) / tiles;
+  if (want > steps / 8) want = steps / 8;
   if (want > 1024) want = 1024;
   if (want < 1) want = 1;
   per = ((steps + want - 1) / want) * kBK;
   splits = int((K + per - 1) / per);
 }
 
-constexpr int64_t kBiasPer = 4096;   // pixels per bias-gradient partial
+constexpr int64_t kBiasPer = 1024;   // pixels per bias-gradient partial
 
 }  // namespace
 
@@ -535,14 +551,14 @@ int gvv_conv_backward_weight(int32_t B, int32_t H, int32_t W, int32_t Cin, int32
   a.k_per_split = per;
   launch_tiled<kBwdWeight>(a, splits, s);
   const int64_t n = int64_t(Cout) * ks * ks * Cin;
-  hipLaunchKernelGGL(k_split_sum, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, workspace, splits, n, dw);
+  hipLaunchKernelGGL(k_split_sum, dim3(unsigned((n + 63) / 64)), dim3(256), 0, s, workspace, splits, n, dw);
   if (db) {
     const int64_t pixels = int64_t(B) * g.OH * g.OW;
     const int bsplits = int((pixels + kBiasPer - 1) / kBiasPer);
     float* bpart = workspace + int64_t(splits) * n;
     hipLaunchKernelGGL(k_bias_partial, dim3(unsigned((Cout + 63) / 64), unsigned(bsplits)), dim3(256), 0, s,
                        static_cast<const uint16_t*>(dz), pixels, Cout, kBiasPer, bpart);
-    hipLaunchKernelGGL(k_split_sum, dim3(unsigned((Cout + 255) / 256)), dim3(256), 0, s, bpart, bsplits,
+    hipLaunchKernelGGL(k_split_sum, dim3(unsigned((Cout + 63) / 64)), dim3(256), 0, s, bpart, bsplits,
                        int64_t(Cout), db);
   }
   return hip_status("conv backward weight");
