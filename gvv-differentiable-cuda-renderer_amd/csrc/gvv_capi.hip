@@ -137,6 +137,7 @@ int plan_forward(Handle* h, int B, int texH, int texW, const float* vpos, const 
   p.uv_table = h->d_uv_table.p;
   p.cams = h->cams.p; p.fnormal = h->fnormal.p; p.tris = h->tris.p; p.proj = h->proj.p;
   p.counters = h->counters.p; p.large = h->large.p; p.large_box = h->large_box;
+  p.mesh_id = 0; p.shared_queue = false;
   p.vnormal_out = vnormal;
   p.normal_map_out = normal_map;
   ResolveArgs& r = p.r;

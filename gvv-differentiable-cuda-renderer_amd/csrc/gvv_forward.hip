@@ -171,8 +171,8 @@ constexpr int kTinyBox = 16;
 __global__ void __launch_bounds__(64) k_raster(const TriSetup* __restrict__ tris, const int32_t* __restrict__ faces,
                                                const float4* __restrict__ proj, const Camera* __restrict__ cams,
                                                unsigned long long* __restrict__ keys, unsigned int* __restrict__ large_count,
-                                               int2* __restrict__ large, int B, int C, int N, int F, int W, int H,
-                                               int large_box) {
+                                               int4* __restrict__ large, int B, int C, int N, int F, int W, int H,
+                                               int large_box, int mesh_id) {
   __shared__ FaceStage st[64];
   __shared__ unsigned cand[kCandCap];
   const int lane = threadIdx.x;
@@ -191,7 +191,7 @@ __global__ void __launch_bounds__(64) k_raster(const TriSetup* __restrict__ tris
       bh = r.bx.y1 - r.bx.y0 + 1;
       if ((int64_t)bw * bh > large_box) {
         const unsigned int slot = atomicAdd(large_count, 1u);
-        large[slot] = make_int2(bc, f);
+        large[slot] = make_int4(bc, f, mesh_id, 0);
       } else {
         small = true;
         FaceStage s;
@@ -245,15 +245,87 @@ __global__ void __launch_bounds__(64) k_raster(const TriSetup* __restrict__ tris
   flush_candidates(st, cand, count, cams, one_view, bc0, keys, W, H);
 }
 
+// One large face (bc, f) rasterised by the whole wavefront.  Instead of the whole bounding box
+// the wavefront walks, 64 rows at a time, the conservative span of each row (row_span; a wave
+// prefix sum turns spans into pixel indices), applies the edge pre-test, compacts and runs the
+// exact test as k_raster does.
+__device__ __forceinline__ void raster_large_face(const TriSetup* __restrict__ tris, const int32_t* __restrict__ faces,
+                                                  const float4* __restrict__ proj, const Camera* __restrict__ cams,
+                                                  unsigned long long* __restrict__ keys, int C, int N, int F, int W,
+                                                  int H, int bc, int f, FaceStage* st, unsigned* cand, int* s_xl,
+                                                  int* s_start) {
+  const RasterFace r = load_face(faces, proj + (size_t)bc * N, f, W, H);
+  const int bw = r.bx.x1 - r.bx.x0 + 1, bh = r.bx.y1 - r.bx.y0 + 1;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    FaceStage s;
+    s.t = tris[(size_t)(bc / C) * F + f];
+    s.z0 = r.p0.z; s.z1 = r.p1.z; s.z2 = r.p2.z;
+    s.f = f; s.x0 = r.bx.x0; s.y0 = r.bx.y0; s.bw = bw; s.bc = bc;
+    st[0] = s;
+  }
+  const EdgeTest et = edge_test(r.p0, r.p1, r.p2);
+  __syncthreads();
+  int count = 0;
+  const int lane = threadIdx.x;
+  for (int rbase = 0; rbase < bh; rbase += 64) {
+    // this lane's row: span and length; inclusive scan of the lengths over the wave
+    const int y = r.bx.y0 + rbase + lane;
+    int xl = 0, xr = -1;
+    if (rbase + lane < bh) row_span(et, y, r.bx.x0, r.bx.x1, xl, xr);
+    const int len = xr >= xl ? xr - xl + 1 : 0;
+    int incl = len;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int t = __shfl_up(incl, d, 64);
+      if (lane >= d) incl += t;
+    }
+    const int total = __shfl(incl, 63, 64);
+    if (total == 0) continue;
+    __syncthreads();
+    s_xl[lane] = xl;
+    s_start[lane] = incl - len;
+    __syncthreads();
+    for (int q0 = 0; q0 < total; q0 += 64) {
+      if (count > kCandCap - 64) {
+        __syncthreads();
+        flush_candidates(st, cand, count, cams, true, bc, keys, W, H);
+        __syncthreads();
+        count = 0;
+      }
+      const int q = q0 + lane;
+      bool pass = false;
+      int p = 0;
+      if (q < total) {
+        // the row holding span pixel q: the last row whose start is <= q
+        int row = 0;
+#pragma unroll
+        for (int step = 32; step > 0; step >>= 1)
+          if (row + step < 64 && s_start[row + step] <= q) row += step;
+        const int x = s_xl[row] + (q - s_start[row]);
+        const int yy = r.bx.y0 + rbase + row;
+        pass = et.pass(x + 0.5f, yy + 0.5f);
+        p = (yy - r.bx.y0) * bw + (x - r.bx.x0);
+      }
+      const uint64_t m = __ballot(pass);
+      if (pass) {
+        const int off = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+        cand[count + off] = (unsigned)p;
+      }
+      count += __popcll(m);
+    }
+  }
+  __syncthreads();
+  flush_candidates(st, cand, count, cams, true, bc, keys, W, H);
+}
+
 // Persistent wavefronts drain the large-face queue: each takes one face at a time from an
-// atomic cursor.  Instead of the whole bounding box the wavefront walks, 64 rows at a time, the
-// conservative span of each row (row_span; a wave prefix sum turns spans into pixel indices),
-// applies the edge pre-test, compacts and runs the exact test as k_raster does.  Every
-// wavefront leaves when the cursor passes the queue length written by k_raster.
+// atomic cursor and rasterises it (raster_large_face).  Every wavefront leaves when the
+// cursor passes the queue length written by k_raster.
 __global__ void __launch_bounds__(64) k_raster_large(const TriSetup* __restrict__ tris, const int32_t* __restrict__ faces,
                                                      const float4* __restrict__ proj, const Camera* __restrict__ cams,
                                                      unsigned long long* __restrict__ keys,
-                                                     unsigned int* __restrict__ counters, const int2* __restrict__ large,
+                                                     unsigned int* __restrict__ counters, const int4* __restrict__ large,
                                                      int C, int N, int F, int W, int H) {
   __shared__ FaceStage st[1];
   __shared__ unsigned cand[kCandCap];
@@ -264,70 +336,29 @@ __global__ void __launch_bounds__(64) k_raster_large(const TriSetup* __restrict_
     if (threadIdx.x == 0) item = atomicAdd(&counters[1], 1u);
     item = __builtin_amdgcn_readfirstlane(item);
     if (item >= n_large) break;
-    const int bc = large[item].x, f = large[item].y;
-    const RasterFace r = load_face(faces, proj + (size_t)bc * N, f, W, H);
-    const int bw = r.bx.x1 - r.bx.x0 + 1, bh = r.bx.y1 - r.bx.y0 + 1;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      FaceStage s;
-      s.t = tris[(size_t)(bc / C) * F + f];
-      s.z0 = r.p0.z; s.z1 = r.p1.z; s.z2 = r.p2.z;
-      s.f = f; s.x0 = r.bx.x0; s.y0 = r.bx.y0; s.bw = bw; s.bc = bc;
-      st[0] = s;
-    }
-    const EdgeTest et = edge_test(r.p0, r.p1, r.p2);
-    __syncthreads();
-    int count = 0;
-    const int lane = threadIdx.x;
-    for (int rbase = 0; rbase < bh; rbase += 64) {
-      // this lane's row: span and length; inclusive scan of the lengths over the wave
-      const int y = r.bx.y0 + rbase + lane;
-      int xl = 0, xr = -1;
-      if (rbase + lane < bh) row_span(et, y, r.bx.x0, r.bx.x1, xl, xr);
-      const int len = xr >= xl ? xr - xl + 1 : 0;
-      int incl = len;
-#pragma unroll
-      for (int d = 1; d < 64; d <<= 1) {
-        const int t = __shfl_up(incl, d, 64);
-        if (lane >= d) incl += t;
-      }
-      const int total = __shfl(incl, 63, 64);
-      if (total == 0) continue;
-      __syncthreads();
-      s_xl[lane] = xl;
-      s_start[lane] = incl - len;
-      __syncthreads();
-      for (int q0 = 0; q0 < total; q0 += 64) {
-        if (count > kCandCap - 64) {
-          __syncthreads();
-          flush_candidates(st, cand, count, cams, true, bc, keys, W, H);
-          __syncthreads();
-          count = 0;
-        }
-        const int q = q0 + lane;
-        bool pass = false;
-        int p = 0;
-        if (q < total) {
-          // the row holding span pixel q: the last row whose start is <= q
-          int row = 0;
-#pragma unroll
-          for (int step = 32; step > 0; step >>= 1)
-            if (row + step < 64 && s_start[row + step] <= q) row += step;
-          const int x = s_xl[row] + (q - s_start[row]);
-          const int yy = r.bx.y0 + rbase + row;
-          pass = et.pass(x + 0.5f, yy + 0.5f);
-          p = (yy - r.bx.y0) * bw + (x - r.bx.x0);
-        }
-        const uint64_t m = __ballot(pass);
-        if (pass) {
-          const int off = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
-          cand[count + off] = (unsigned)p;
-        }
-        count += __popcll(m);
-      }
-    }
-    __syncthreads();
-    flush_candidates(st, cand, count, cams, true, bc, keys, W, H);
+    raster_large_face(tris, faces, proj, cams, keys, C, N, F, W, H, large[item].x, large[item].y, st, cand, s_xl,
+                      s_start);
+  }
+}
+
+// The same over a queue shared by several meshes: each entry names its mesh, whose raster
+// state comes from the descriptor table.  One launch drains the large faces of every mesh.
+__global__ void __launch_bounds__(64) k_raster_large_multi(const RasterMesh* __restrict__ meshes,
+                                                           unsigned int* __restrict__ counters,
+                                                           const int4* __restrict__ large) {
+  __shared__ FaceStage st[1];
+  __shared__ unsigned cand[kCandCap];
+  __shared__ int s_xl[64], s_start[64];
+  const unsigned int n_large = counters[0];
+  for (;;) {
+    unsigned int item = 0;
+    if (threadIdx.x == 0) item = atomicAdd(&counters[1], 1u);
+    item = __builtin_amdgcn_readfirstlane(item);
+    if (item >= n_large) break;
+    const int4 e = large[item];
+    const RasterMesh m = meshes[e.z];
+    raster_large_face(m.tris, m.faces, m.proj, m.cams, m.keys, m.C, m.N, m.F, m.W, m.H, e.x, e.y, st, cand, s_xl,
+                      s_start);
   }
 }
 
@@ -448,7 +479,25 @@ static void launch_resolve_t(const ResolveArgs& a, hipStream_t s) {
   hipLaunchKernelGGL((k_resolve<ALBEDO, SHADE>), dim3(blocks_for(n)), dim3(256), 0, s, a);
 }
 
-hipError_t launch_forward(const ForwardPlan& p, hipStream_t s) {
+RasterMesh raster_mesh(const ForwardPlan& p) {
+  RasterMesh m;
+  m.tris = p.tris; m.faces = p.faces; m.proj = p.proj; m.cams = p.cams; m.keys = p.r.keys;
+  m.C = p.C; m.N = p.N; m.F = p.F; m.W = p.r.W; m.H = p.r.H;
+  return m;
+}
+
+hipError_t launch_clear_counters(unsigned int* counters, hipStream_t s) {
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, counters, 2);
+  return hipGetLastError();
+}
+
+hipError_t launch_raster_large_multi(const RasterMesh* meshes, unsigned int* counters, const int4* large,
+                                     hipStream_t s) {
+  hipLaunchKernelGGL(k_raster_large_multi, dim3(kLargeWorkgroups), dim3(64), 0, s, meshes, counters, large);
+  return hipGetLastError();
+}
+
+hipError_t launch_forward_raster(const ForwardPlan& p, hipStream_t s) {
   const int B = p.B, C = p.C, N = p.N, F = p.F;
   {
     ProfScope ps(p.prof, "k_cameras", s);
@@ -470,17 +519,23 @@ hipError_t launch_forward(const ForwardPlan& p, hipStream_t s) {
     int T = p.r.texW * p.r.texH;
     hipLaunchKernelGGL(k_normal_map, dim3(blocks_for((int64_t)B * T)), dim3(256), 0, s, p.uv_table, p.faces,
                        p.vnormal_out, p.normal_map_out, B, C, N, T);
-  } else {
-    {
-      ProfScope ps(p.prof, "k_raster", s);
-      hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, p.counters, 2);
-      hipLaunchKernelGGL(k_raster, dim3(blocks_for((int64_t)B * C * F, 64)), dim3(64), 0, s, p.tris, p.faces, p.proj,
-                         p.cams, p.r.keys, p.counters, p.large, B, C, N, F, p.r.W, p.r.H, p.large_box);
-    }
+    return hipGetLastError();
+  }
+  {
+    ProfScope ps(p.prof, "k_raster", s);
+    if (!p.shared_queue) hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, p.counters, 2);
+    hipLaunchKernelGGL(k_raster, dim3(blocks_for((int64_t)B * C * F, 64)), dim3(64), 0, s, p.tris, p.faces, p.proj,
+                       p.cams, p.r.keys, p.counters, p.large, B, C, N, F, p.r.W, p.r.H, p.large_box, p.mesh_id);
+  }
+  if (!p.shared_queue) {
     ProfScope ps(p.prof, "k_raster_large", s);
     hipLaunchKernelGGL(k_raster_large, dim3(kLargeWorkgroups), dim3(64), 0, s, p.tris, p.faces, p.proj, p.cams,
                        p.r.keys, p.counters, p.large, C, N, F, p.r.W, p.r.H);
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_forward_resolve(const ForwardPlan& p, hipStream_t s) {
   // In normal-map mode the visibility buffer is empty, so this writes the background.
   ProfScope ps(p.prof, "k_resolve", s);
   switch (p.albedo * 2 + (p.shade ? 1 : 0)) {
@@ -496,6 +551,12 @@ hipError_t launch_forward(const ForwardPlan& p, hipStream_t s) {
     default: launch_resolve_t<kForegroundMask, true>(p.r, s); break;
   }
   return hipGetLastError();
+}
+
+hipError_t launch_forward(const ForwardPlan& p, hipStream_t s) {
+  hipError_t e = launch_forward_raster(p, s);
+  if (e != hipSuccess) return e;
+  return launch_forward_resolve(p, s);
 }
 
 }  // namespace gvv

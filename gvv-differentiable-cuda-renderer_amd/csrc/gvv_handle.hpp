@@ -45,7 +45,7 @@ struct Handle {
   DevBuf<float4> proj;
   DevBuf<unsigned long long> keys;
   DevBuf<unsigned int> counters;
-  DevBuf<int2> large;
+  DevBuf<int4> large;
   DevBuf<float> g_vn;
   DevBuf<float> sh_partial;
   Profiler prof;

@@ -79,6 +79,16 @@ enum Shading { kShaded = 0, kShadeless = 1 };
 // One entry of the texel -> (face, barycentrics) table used by the normal map.
 struct UvTexel { int32_t face; float a, b, c; };
 
+// The raster state of one mesh as the large-face kernel needs it.
+struct RasterMesh {
+  const TriSetup* tris;              // [B][F]
+  const int32_t* faces;              // [F][3]
+  const float4* proj;                // [B][C][N]
+  const Camera* cams;                // [B][C]
+  unsigned long long* keys;          // [B][C][H][W]
+  int C, N, F, W, H;
+};
+
 struct ResolveArgs {
   int B, C, N, W, H, texW, texH;
   unsigned long long* keys;          // [B][C][H][W]; reset to empty after reading
@@ -115,7 +125,10 @@ struct ForwardPlan {
   float* vnormal_out;                // [B][C][N][3]
   float* normal_map_out;             // [B][texH][texW][3]
   unsigned int* counters;            // [2]: large-face queue length, drain cursor
-  int2* large;                       // [B*C*F] large-face queue of (b*C + c, face)
+  int4* large;                       // large-face queue of (b*C + c, face, mesh, 0)
+  int mesh_id;                       // written into the queue entries
+  bool shared_queue;                 // the queue is shared by several meshes: the caller clears
+                                     // the counters and drains it (k_raster_large_multi)
   int large_box;                     // bounding boxes above this many pixels go to the queue
   ResolveArgs r;
   Profiler* prof;
@@ -162,6 +175,15 @@ hipError_t launch_cameras(const float* extr, const float* intr, Camera* cams, in
 hipError_t launch_face_setup(const float* vpos, const int32_t* faces, float* fnormal, TriSetup* tris, int B, int N,
                              int F, hipStream_t s);
 hipError_t launch_forward(const ForwardPlan& p, hipStream_t s);
+// The two halves of launch_forward: up to and including the depth pass (with the large-face
+// queue unless the plan's queue is shared), then resolve.
+hipError_t launch_forward_raster(const ForwardPlan& p, hipStream_t s);
+hipError_t launch_forward_resolve(const ForwardPlan& p, hipStream_t s);
+RasterMesh raster_mesh(const ForwardPlan& p);
+hipError_t launch_clear_counters(unsigned int* counters, hipStream_t s);
+// Drains a large-face queue shared by several meshes (entries carry the mesh index).
+hipError_t launch_raster_large_multi(const RasterMesh* meshes, unsigned int* counters, const int4* large,
+                                     hipStream_t s);
 hipError_t launch_backward(const BackwardPlan& p, hipStream_t s);
 
 }  // namespace gvv

@@ -1,12 +1,17 @@
 // gvv_multi.hip -- multi-mesh runtime: many meshes of uneven size in one hipGraph launch, with
 // an autotuner over the per-mesh launch settings (C ABI in include/gvv_multi.h).
 //
-// Each mesh keeps its own renderer handle (topology, CSR, scratch), so meshes share no
-// buffers and their kernel chains are independent.  A run plans every mesh on the host (this
-// is where scratch grows, outside any capture), then either launches the chains in order or
-// captures them once into a hipGraph whose branches -- one per hardware queue -- take the
-// meshes longest-first (LPT), so small meshes run beside large ones instead of after them.
-// The graph is replayed while the io table and the settings are unchanged.
+// Each mesh keeps its own renderer handle (topology, CSR, scratch).  The one thing the meshes
+// share is the large-face queue: every mesh's depth pass (k_raster) appends its faces with big
+// bounding boxes, tagged with the mesh index, to one queue, and one persistent launch
+// (k_raster_large_multi) drains the large faces of all meshes together, so a mesh with few
+// large faces leaves no idle tail and the launch count does not grow with the mesh count.
+// A run plans every mesh on the host (this is where scratch grows, outside any capture), then
+// either launches the stages in order or captures them once into a hipGraph of three stages:
+// the per-mesh depth passes on parallel branches (one per hardware queue, meshes
+// longest-first onto the least-loaded branch, LPT), the shared large-face launch, then the
+// per-mesh resolve and backward chains on the branches again.  The graph is replayed while
+// the io table and the settings are unchanged.
 //
 // The reference renders one topology per op and serialises the batch on the host
 // (cpp/src/TensorflowOperators/CudaRenderer/CudaRenderer.cpp:23, :309-328); nothing here has a
@@ -52,8 +57,13 @@ struct Multi {
   int albedo = 0, shading = 0, device = 0;
   bool use_graph = true;
   hipStream_t lanes[kLanes] = {};
-  hipEvent_t fork = nullptr;
-  hipEvent_t join[kLanes] = {};
+  hipEvent_t fork[2] = {};
+  hipEvent_t join[2][kLanes] = {};
+  // the shared large-face queue and the raster state of every mesh it names
+  DevBuf<unsigned int> counters;                // [2]: queue length, drain cursor
+  DevBuf<int4> large;                           // sum over meshes of B*C*F entries
+  DevBuf<RasterMesh> table;                     // [meshes]
+  std::vector<RasterMesh> host_table;           // what `table` holds
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   std::vector<char> key;      // what the graph was captured for
@@ -70,9 +80,14 @@ struct Multi {
     drop_graph();
     for (auto s : lanes)
       if (s) (void)hipStreamDestroy(s);
-    if (fork) (void)hipEventDestroy(fork);
-    for (auto e : join)
-      if (e) (void)hipEventDestroy(e);
+    for (int st = 0; st < 2; ++st) {
+      if (fork[st]) (void)hipEventDestroy(fork[st]);
+      for (auto e : join[st])
+        if (e) (void)hipEventDestroy(e);
+    }
+    counters.release();
+    large.release();
+    table.release();
     for (auto* h : meshes) gvv_destroy(h);
   }
 };
@@ -102,6 +117,12 @@ hipError_t enqueue_mesh(const ForwardPlan& fp, const BackwardPlan& bp, bool back
   return e;
 }
 
+hipError_t enqueue_tail(const ForwardPlan& fp, const BackwardPlan& bp, bool backward, hipStream_t s) {
+  hipError_t e = launch_forward_resolve(fp, s);
+  if (e == hipSuccess && backward) e = launch_backward(bp, s);
+  return e;
+}
+
 int check_device(const Multi* m) {
   int dev = -1;
   MULTI_CHECK(hipGetDevice(&dev));
@@ -109,31 +130,93 @@ int check_device(const Multi* m) {
   return 0;
 }
 
+// Plans every mesh onto the shared large-face queue and brings the device copy of the mesh
+// table up to date (on `s`, after the work already queued there, which may still read it).
 int plan_all(Multi* m, const gvv_mesh_io* io, bool backward, hipStream_t s, std::vector<ForwardPlan>& fp,
              std::vector<BackwardPlan>& bp) {
   const int n = (int)m->meshes.size();
   fp.resize(n);
   bp.resize(n);
-  for (int i = 0; i < n; ++i)
+  size_t queue = 0;
+  for (int i = 0; i < n; ++i) {
     if (plan_mesh(m, i, io[i], backward, s, fp[i], bp[i])) return 1;
+    queue += (size_t)io[i].batch * m->meshes[i]->C * m->meshes[i]->F;
+  }
+  if (m->large.n < queue || !m->counters.p || !m->table.p) MULTI_CHECK(hipStreamSynchronize(s));
+  MULTI_CHECK(m->counters.reserve(2));
+  MULTI_CHECK(m->large.reserve(queue));
+  MULTI_CHECK(m->table.reserve(n));
+  std::vector<RasterMesh> t(n);
+  for (int i = 0; i < n; ++i) {
+    fp[i].counters = m->counters.p;
+    fp[i].large = m->large.p;
+    fp[i].mesh_id = i;
+    fp[i].shared_queue = true;
+    t[i] = raster_mesh(fp[i]);
+  }
+  if (m->host_table.size() != t.size() ||
+      memcmp(m->host_table.data(), t.data(), t.size() * sizeof(RasterMesh)) != 0) {
+    // the previous contents may still be read by queued work; the copy source must outlive it
+    MULTI_CHECK(hipStreamSynchronize(s));
+    m->host_table = t;
+    MULTI_CHECK(hipMemcpyAsync(m->table.p, m->host_table.data(), t.size() * sizeof(RasterMesh),
+                               hipMemcpyHostToDevice, s));
+  }
   return 0;
+}
+
+// Every launch of a run in stream order on one stream: clear the queue, the depth passes, the
+// shared large-face launch, then each mesh's resolve and backward.
+hipError_t enqueue_all(Multi* m, const std::vector<ForwardPlan>& fp, const std::vector<BackwardPlan>& bp,
+                       bool backward, hipStream_t s) {
+  hipError_t e = launch_clear_counters(m->counters.p, s);
+  for (size_t i = 0; i < fp.size() && e == hipSuccess; ++i) e = launch_forward_raster(fp[i], s);
+  if (e == hipSuccess) e = launch_raster_large_multi(m->table.p, m->counters.p, m->large.p, s);
+  for (size_t i = 0; i < fp.size() && e == hipSuccess; ++i) e = enqueue_tail(fp[i], bp[i], backward, s);
+  return e;
 }
 
 std::vector<char> graph_key(const Multi* m, const gvv_mesh_io* io, bool backward) {
   const size_t n = m->meshes.size();
-  std::vector<char> k(n * sizeof(gvv_mesh_io) + 1 + n * 2 * sizeof(int));
+  const size_t head = n * sizeof(gvv_mesh_io) + 1;
+  std::vector<char> k(head + n * 2 * sizeof(int) + 3 * sizeof(void*));
   memcpy(k.data(), io, n * sizeof(gvv_mesh_io));
   k[n * sizeof(gvv_mesh_io)] = backward ? 1 : 0;
-  int* t = reinterpret_cast<int*>(k.data() + n * sizeof(gvv_mesh_io) + 1);
+  char* t = k.data() + head;
   for (size_t i = 0; i < n; ++i) {
-    memcpy(t + 2 * i, &m->meshes[i]->tile, sizeof(int));
-    memcpy(t + 2 * i + 1, &m->meshes[i]->large_box, sizeof(int));
+    memcpy(t + 8 * i, &m->meshes[i]->tile, sizeof(int));
+    memcpy(t + 8 * i + 4, &m->meshes[i]->large_box, sizeof(int));
   }
+  // the shared queue's buffers are baked into the graph
+  const void* bufs[3] = {m->counters.p, m->large.p, m->table.p};
+  memcpy(t + 8 * n, bufs, sizeof(bufs));
   return k;
 }
 
-// Captures every mesh's chain into one graph: fork from lane 0, meshes longest-first onto the
-// least-loaded lane, join back into lane 0.
+// Fans the meshes out over the lanes (longest first onto the least-loaded lane), each lane
+// starting after event `fork` on lane 0 and joining back into lane 0.
+template <class Enqueue>
+hipError_t fan_out(Multi* m, const std::vector<int>& order, const gvv_mesh_io* io, int stage, Enqueue&& enqueue) {
+  hipError_t e = hipEventRecord(m->fork[stage], m->lanes[0]);
+  for (int k = 1; k < kLanes && e == hipSuccess; ++k) e = hipStreamWaitEvent(m->lanes[k], m->fork[stage], 0);
+  double load[kLanes] = {0.0, 0.0, 0.0, 0.0};
+  for (size_t j = 0; j < order.size() && e == hipSuccess; ++j) {
+    const int i = order[j];
+    int lane = 0;
+    for (int k = 1; k < kLanes; ++k)
+      if (load[k] < load[lane]) lane = k;
+    load[lane] += mesh_cost(m->meshes[i], io[i].batch);
+    e = enqueue(i, m->lanes[lane]);
+  }
+  for (int k = 1; k < kLanes && e == hipSuccess; ++k) {
+    e = hipEventRecord(m->join[stage][k], m->lanes[k]);
+    if (e == hipSuccess) e = hipStreamWaitEvent(m->lanes[0], m->join[stage][k], 0);
+  }
+  return e;
+}
+
+// Captures a run into one graph: clear the queue; depth passes fanned out over the lanes;
+// the shared large-face launch; resolve + backward fanned out again.
 int capture(Multi* m, const gvv_mesh_io* io, bool backward, const std::vector<ForwardPlan>& fp,
             const std::vector<BackwardPlan>& bp) {
   const int n = (int)m->meshes.size();
@@ -144,21 +227,12 @@ int capture(Multi* m, const gvv_mesh_io* io, bool backward, const std::vector<Fo
   });
   m->drop_graph();
   MULTI_CHECK(hipStreamBeginCapture(m->lanes[0], hipStreamCaptureModeThreadLocal));
-  hipError_t e = hipEventRecord(m->fork, m->lanes[0]);
-  for (int k = 1; k < kLanes && e == hipSuccess; ++k) e = hipStreamWaitEvent(m->lanes[k], m->fork, 0);
-  double load[kLanes] = {0.0, 0.0, 0.0, 0.0};
-  for (int j = 0; j < n && e == hipSuccess; ++j) {
-    const int i = order[j];
-    int lane = 0;
-    for (int k = 1; k < kLanes; ++k)
-      if (load[k] < load[lane]) lane = k;
-    load[lane] += mesh_cost(m->meshes[i], io[i].batch);
-    e = enqueue_mesh(fp[i], bp[i], backward, m->lanes[lane]);
-  }
-  for (int k = 1; k < kLanes && e == hipSuccess; ++k) {
-    e = hipEventRecord(m->join[k], m->lanes[k]);
-    if (e == hipSuccess) e = hipStreamWaitEvent(m->lanes[0], m->join[k], 0);
-  }
+  hipError_t e = launch_clear_counters(m->counters.p, m->lanes[0]);
+  if (e == hipSuccess)
+    e = fan_out(m, order, io, 0, [&](int i, hipStream_t s) { return launch_forward_raster(fp[i], s); });
+  if (e == hipSuccess) e = launch_raster_large_multi(m->table.p, m->counters.p, m->large.p, m->lanes[0]);
+  if (e == hipSuccess)
+    e = fan_out(m, order, io, 1, [&](int i, hipStream_t s) { return enqueue_tail(fp[i], bp[i], backward, s); });
   hipGraph_t g = nullptr;
   const hipError_t e_end = hipStreamEndCapture(m->lanes[0], &g);
   if (e != hipSuccess || e_end != hipSuccess) {
@@ -199,8 +273,11 @@ int gvv_multi_create(const gvv_mesh_desc* meshes, int32_t n, int32_t albedo_mode
   }
   hipError_t e = hipGetDevice(&m->device);
   for (int k = 0; k < kLanes && e == hipSuccess; ++k) e = hipStreamCreateWithFlags(&m->lanes[k], hipStreamNonBlocking);
-  if (e == hipSuccess) e = hipEventCreateWithFlags(&m->fork, hipEventDisableTiming);
-  for (int k = 0; k < kLanes && e == hipSuccess; ++k) e = hipEventCreateWithFlags(&m->join[k], hipEventDisableTiming);
+  for (int st = 0; st < 2; ++st) {
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&m->fork[st], hipEventDisableTiming);
+    for (int k = 0; k < kLanes && e == hipSuccess; ++k)
+      e = hipEventCreateWithFlags(&m->join[st][k], hipEventDisableTiming);
+  }
   if (e != hipSuccess) {
     delete m;
     return fail(std::string("gvv_multi_create: ") + hipGetErrorString(e));
@@ -230,7 +307,7 @@ int gvv_multi_run(void* handle, const gvv_mesh_io* io, int32_t backward, void* s
   std::vector<BackwardPlan> bp;
   if (plan_all(m, io, backward != 0, s, fp, bp)) return 1;
   if (!m->use_graph) {
-    for (size_t i = 0; i < fp.size(); ++i) MULTI_CHECK(enqueue_mesh(fp[i], bp[i], backward != 0, s));
+    MULTI_CHECK(enqueue_all(m, fp, bp, backward != 0, s));
     return 0;
   }
   if (!m->exec || m->key != graph_key(m, io, backward != 0))
