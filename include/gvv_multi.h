@@ -55,8 +55,10 @@ void gvv_multi_destroy(void* handle);
 /* Forward of every mesh, then (if backward != 0) the backward of every mesh against
  * io[i].g_render.  With graphs on (the default) all launches of all meshes are captured once
  * into one hipGraph, meshes spread over parallel branches, and replayed on `stream` while the
- * io table stays the same; a changed table or tuning is captured again.  Results equal those
- * of gvv_forward / gvv_backward on each mesh with the same tuning. */
+ * io table stays the same; a changed table or tuning is captured again.  The large faces of
+ * all meshes go to one queue that one persistent launch drains, so runs of one handle must not
+ * overlap on different streams.  Results equal those of gvv_forward / gvv_backward on each mesh
+ * with the same tuning. */
 int gvv_multi_run(void* handle, const gvv_mesh_io* io, int32_t backward, void* stream);
 
 /* 1: hipGraph replay (default); 0: plain launches of every mesh in order on `stream`. */
