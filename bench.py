@@ -40,7 +40,10 @@ for _p in (PKG, ROOT):
         sys.path.insert(0, _p)
 
 HBM_PEAK_GBS = 8000.0
-VALU_PEAK_PER_S = 157.3e12 / 2 / 64     # wave64 FP32 FMA instructions/s at the 157.3 TFLOP/s vector peak
+# wave64 VALU instructions/s: 256 CUs x 4 SIMDs x 2.4 GHz, one wave64 instruction per 4 cycles on
+# a 16-lane SIMD (78.6 TFLOP/s of FP32 FMAs; the 157.3 TFLOP/s FP32 figure counts packed-FP32
+# instructions, two FMAs per lane, which this library is built without)
+VALU_PEAK_PER_S = 256 * 4 * 2.4e9 / 4
 
 
 def kernel_bytes(name, scene, covered, texW, texH):
@@ -345,7 +348,7 @@ def main():
             with open(vf) as fh:
                 insts = json.load(fh).get(dom)
             if insts:
-                # share of the wave64 VALU issue rate at the FP32 vector peak
+                # share of the wave64 VALU issue rate
                 roofline["valu_issue_frac"] = round(insts / (dom_ms / 1e3 * VALU_PEAK_PER_S), 4)
 
     atomics = None
