@@ -37,6 +37,25 @@ __device__ __forceinline__ float wave_total(float v) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 
+// Merges the per-face terms of lane pairs (lane, lane ^ MASK) that add to the same face slot:
+// the lower lane takes the sum and the upper one drops out (slot -1), so fewer lanes add to
+// one LDS address in the atomics that follow (same-address atomics serialise).  UP is the DPP
+// control by which a lower lane reads lane ^ MASK, DOWN the one by which an upper lane does
+// (the same control for the self-inverse permutations); every lane must be active.
+template <int UP, int DOWN, int MASK, int NT>
+__device__ __forceinline__ void merge_pairs(float* t, int& slot, int lane) {
+  const bool low = (lane & MASK) == 0;
+  const int up = __builtin_amdgcn_update_dpp(-1, slot, UP, 0xf, 0xf, false);
+  const int down = UP == DOWN ? up : __builtin_amdgcn_update_dpp(-1, slot, DOWN, 0xf, 0xf, false);
+  const bool merge = slot >= 0 && (low ? up : down) == slot;
+#pragma unroll
+  for (int k = 0; k < NT; ++k) {
+    const float w = dpp_mov<UP, 0xf, false>(t[k]);
+    if (merge && low) t[k] += w;
+  }
+  if (merge && !low) slot = -1;
+}
+
 // Double-precision 3-vector for the barycentric Jacobian below.
 struct D3 { double x, y, z; };
 __device__ __forceinline__ D3 d3(double x, double y, double z) { D3 r; r.x = x; r.y = y; r.z = z; return r; }
@@ -67,15 +86,34 @@ __device__ __forceinline__ D3 cross(D3 a, D3 b) { return d3(a.y * b.z - a.z * b.
 // as 2.8e-4 max-norm position-gradient error on a 5k-face sphere), so those two are formed
 // in FP64 from exact vertex differences; everything after them is well conditioned and stays
 // binary32 (1.3e-7 max-norm against the FP64 oracle on the same sphere).
+// Arithmetic of continuous gradient terms.  Binary32 rounding is part of the contract only for
+// the visibility and decision values (barycentrics, texel indices, the normal flip, early
+// exits); terms that only flow into gradients are checked against the FP64 oracle with a
+// relative tolerance, so in this type's operations products and sums may fuse into FMAs.
+#pragma clang fp contract(fast)
+struct F3 { float x, y, z; };
+GVV_HD F3 f3(V3 v) { F3 r; r.x = v.x; r.y = v.y; r.z = v.z; return r; }
+GVV_HD F3 f3(float x, float y, float z) { F3 r; r.x = x; r.y = y; r.z = z; return r; }
+GVV_HD F3 operator+(F3 a, F3 b) { return f3(a.x + b.x, a.y + b.y, a.z + b.z); }
+GVV_HD F3 operator-(F3 a, F3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
+GVV_HD F3 operator-(F3 a) { return f3(-a.x, -a.y, -a.z); }
+GVV_HD F3 operator*(float s, F3 a) { return f3(s * a.x, s * a.y, s * a.z); }
+GVV_HD F3 operator*(F3 a, float s) { return f3(s * a.x, s * a.y, s * a.z); }
+GVV_HD float dot(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+GVV_HD F3 cross(F3 a, F3 b) { return f3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+
 template <int OFF>
 __device__ __forceinline__ void bary_position_grad(V3 of, V3 d, V3 v0f, V3 v1f, V3 v2f, V3 gb, float* out) {
-  const V3 e1 = v1f - v0f, e2 = v2f - v0f;
-  const V3 N = cross(e1, e2);
   {
+    // the reference's binary32 early exit (a decision)
+    const V3 e1 = v1f - v0f, e2 = v2f - v0f;
+    const V3 N = cross(e1, e2);
     const float denf = dot(N, N);
     if (fabsf(dot(normalize(d), normalize(N))) < 0.001f || fabsf(denf * denf) < 0.001f) return;
   }
-  V3 q0;
+  const F3 e1 = f3(v1f - v0f), e2 = f3(v2f - v0f), df = f3(d);
+  const F3 N = cross(e1, e2);
+  F3 q0;
   float nd;
   {
     const D3 v0 = d3(v0f), dd = d3(d);
@@ -83,99 +121,141 @@ __device__ __forceinline__ void bary_position_grad(V3 of, V3 d, V3 v0f, V3 v1f, 
     const D3 w = v0 - d3(of);
     const double ndd = dot(Nd, dd);
     const D3 q = (dot(w, Nd) / ndd) * dd - w;
-    q0 = v3((float)q.x, (float)q.y, (float)q.z);
+    q0 = f3((float)q.x, (float)q.y, (float)q.z);
     nd = (float)ndd;
   }
   const float den = dot(N, N);
-  const V3 q1 = q0 - e1, q2 = q0 - e2;
-  const V3 E0 = e2 - e1, E1 = -e2;
+  const F3 q1 = q0 - e1, q2 = q0 - e2;
+  const F3 E0 = e2 - e1, E1 = -e2;
   const float w0 = gb.x - gb.z, w1 = gb.y - gb.z;
-  const V3 C0 = cross(E0, q1), C1 = cross(E1, q2);
-  const V3 Cw = w0 * C0 + w1 * C1;
+  const F3 C0 = cross(E0, q1), C1 = cross(E1, q2);
+  const F3 Cw = w0 * C0 + w1 * C1;
   const float NCw = w0 * dot(N, C0) + w1 * dot(N, C1);
-  const float NEdw = w0 * dot(N, cross(E0, d)) + w1 * dot(N, cross(E1, d));
-  const float inv_den = __builtin_amdgcn_rcpf(den), inv_nd = __builtin_amdgcn_rcpf(nd);   // 1 ulp: continuous terms
+  const float NEdw = w0 * dot(N, cross(E0, df)) + w1 * dot(N, cross(E1, df));
+  const float inv_den = __builtin_amdgcn_rcpf(den), inv_nd = __builtin_amdgcn_rcpf(nd);
   const float c2 = -2.f * NCw * inv_den * inv_den;
 #pragma unroll
   for (int v = 0; v < 3; ++v) {
-    const V3 A = v == 0 ? e1 - e2 : (v == 1 ? e2 : -e1);
-    V3 dt = cross(A, q0) * (-inv_nd);
+    const F3 A = v == 0 ? e1 - e2 : (v == 1 ? e2 : -e1);
+    F3 dt = cross(A, q0) * (-inv_nd);
     if (v == 0) dt = dt + N * inv_nd;
-    V3 Q = v == 0 ? cross(w1 * q2, N) : (v == 1 ? cross(-w0 * q1, N) : cross(w0 * q1 - w1 * q2, N));
+    F3 Q = v == 0 ? cross(w1 * q2, N) : (v == 1 ? cross(-w0 * q1, N) : cross(w0 * q1 - w1 * q2, N));
     if (v == 1) Q = Q - w0 * cross(N, E0);
     if (v == 2) Q = Q - w1 * cross(N, E1);
-    const V3 g = inv_den * (cross(A, Cw) + Q + NEdw * dt) + c2 * cross(A, N);
+    const F3 g = inv_den * (cross(A, Cw) + Q + NEdw * dt) + c2 * cross(A, N);
     out[OFF + 3 * v + 0] += g.x;
     out[OFF + 3 * v + 1] += g.y;
     out[OFF + 3 * v + 2] += g.z;
   }
 }
+// SH basis of a unit normal (the reference's order of the nine terms) and the illumination
+// it gives under the coefficients of one view, for the continuous terms of the backward.
+GVV_HD void sh_basis(V3 n, float Y[9]) {
+  Y[0] = 1.f; Y[1] = n.y; Y[2] = n.z; Y[3] = n.x; Y[4] = n.x * n.y; Y[5] = n.z * n.y;
+  Y[6] = 3.f * n.z * n.z - 1.f; Y[7] = n.x * n.z; Y[8] = n.x * n.x - n.y * n.y;
+}
+GVV_HD V3 sh_light(const float Y[9], const float* sh) {
+  float L[3];
+#pragma unroll
+  for (int ch = 0; ch < 3; ++ch) {
+    float l = 0.f;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) l += sh[9 * ch + k] * Y[k];
+    L[ch] = l;
+  }
+  return v3(L[0], L[1], L[2]);
+}
+// The reference's four-texel blend of the backward's albedo (Grad.cu:282-312).
+GVV_HD V3 blend4(float su, float sv, float tu, float tv, V3 cLL, V3 cHL, V3 cLH, V3 cHH) {
+  const F3 r = sv * (su * f3(cLL) + tu * f3(cHL)) + tv * (su * f3(cLH) + tu * f3(cHH));
+  return v3(r.x, r.y, r.z);
+}
+#pragma clang fp contract(off)
 
 // Per-pixel gradient terms of one covered pixel (reference Grad.cu:205-555).  Adds the SH
-// terms to sh_acc and the texture terms to g_tex directly; returns the per-face terms
-// (position, vertex-normal gradient and vertex-colour rows for the three corners).
+// terms to sh_acc; returns the per-face terms (position, vertex-normal gradient and
+// vertex-colour rows for the three corners) in t and, textured, the texture-gradient term in
+// tex_val for texel tex_key of the view's batch entry (left at -1 when there is none).
+// `cam` and `shc` are the view's camera and SH coefficients, held in registers by the caller.
 template <int ALBEDO, bool SHADED, bool M2D, bool HAVE_BARY, int G_POS, int G_VN, int G_COL>
-__device__ __forceinline__ void pixel_terms(const BackwardArgs& a, int bc, int b, int px, int py, size_t idx, int idf,
-                                            int i0, int i1, int i2, float* t, float sh_acc[27]) {
-  const Camera& cam = a.cams[bc];
-  const V3 o = cam_origin(cam);
-  const V3 d = pixel_ray(cam, px + 0.5f, py + 0.5f);
+__device__ __forceinline__ void pixel_terms(const BackwardArgs& a, const Camera& cam, const float* shc, int bc, int b,
+                                            int px, int py, size_t idx, int idf, int i0, int i1, int i2, float* t,
+                                            float sh_acc[27], int& tex_key, float tex_val[3]) {
+  // Every load that does not depend on the barycentrics is issued before they are computed,
+  // so that all of them share one round trip to memory; only the texels wait for a second.
   const float* vp = a.vpos + (size_t)b * a.N * 3;
   const V3 p0 = ld3(vp + 3 * i0), p1 = ld3(vp + 3 * i1), p2 = ld3(vp + 3 * i2);
-  V3 abc;
-  if (HAVE_BARY) {
-    const float ba = a.bary[2 * idx], bb = a.bary[2 * idx + 1];
-    abc = v3(ba, bb, 1.f - ba - bb);
-  } else {
-    abc = ray_barycentric(tri_setup(p0, p1, p2), cam_origin_scaled(cam), d);
-  }
-  const float w3[3] = {abc.x, abc.y, abc.z};
   const float* vnb = a.vnormal + (size_t)bc * a.N * 3;
   const V3 n0 = ld3(vnb + 3 * i0), n1 = ld3(vnb + 3 * i1), n2 = ld3(vnb + 3 * i2);
+  const float* gr = a.g_render + 3 * idx;
+  const V3 g = v3(gr[0], gr[1], gr[2]);
+  float uv6[6];
+  if (ALBEDO == kTextured) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) uv6[k] = a.uv[6 * (size_t)idf + k];
+  }
+  V3 c0, c1, c2;
+  if (ALBEDO == kVertexColor) {
+    const float* vc = a.vcol + (size_t)b * a.N * 3;
+    c0 = ld3(vc + 3 * i0); c1 = ld3(vc + 3 * i1); c2 = ld3(vc + 3 * i2);
+  }
+  float ba = 0.f, bb = 0.f;
+  if (HAVE_BARY) { ba = a.bary[2 * idx]; bb = a.bary[2 * idx + 1]; }
+
+  const V3 o = cam_origin(cam);
+  const V3 d = pixel_ray(cam, px + 0.5f, py + 0.5f);
+  V3 abc;
+  if (HAVE_BARY) abc = v3(ba, bb, 1.f - ba - bb);
+  else abc = ray_barycentric(tri_setup(p0, p1, p2), cam_origin_scaled(cam), d);
+  const float w3[3] = {abc.x, abc.y, abc.z};
+  // textured: the four texels of the backward's blend (Grad.cu:282-312; not the forward's
+  // nearest texel) are loaded before the normal and the illumination are formed, which do not
+  // need them
+  TexCoord tc;
+  V3 cLL, cHL, cLH, cHH;
+  if (ALBEDO == kTextured) {
+    tc = tex_coord(uv6, abc, a.texW, a.texH);
+    const float* tb = a.tex + (size_t)b * a.texH * a.texW * 3;
+    const int lu = min(max(tc.LU, 0), a.texW - 1), hu = min(max(tc.LU + 1, 0), a.texW - 1);
+    const int lv = min(max(tc.LV, 0), a.texH - 1), hv = min(max(tc.LV + 1, 0), a.texH - 1);
+    cLL = ld3(tb + 3 * ((size_t)lv * a.texW + lu)); cHL = ld3(tb + 3 * ((size_t)lv * a.texW + hu));
+    cLH = ld3(tb + 3 * ((size_t)hv * a.texW + lu)); cHH = ld3(tb + 3 * ((size_t)hv * a.texW + hu));
+  }
   const V3 un = n0 * abc.x + n1 * abc.y + n2 * abc.z;
   const float unl = length(un);
   V3 n = un / unl;         // IEEE division: the flip decision below is part of the binary32 contract
   const bool flipped = dot(n, d) > 0.f;
   if (flipped) n = -n;
-  const float* shc = a.sh + (size_t)bc * 27;
-  const float* gr = a.g_render + 3 * idx;
-  const V3 g = v3(gr[0], gr[1], gr[2]);
 
   V3 L = v3(1.f, 1.f, 1.f);
-  if (SHADED) L = sh_illum(n, shc);
+  float Y[9];
+  if (SHADED) {
+    sh_basis(n, Y);
+    L = sh_light(Y, shc);
+  }
 
   V3 alb = v3(0.f, 0.f, 0.f);
   if (ALBEDO == kVertexColor) {
-    const float* vc = a.vcol + (size_t)b * a.N * 3;
-    const V3 c0 = ld3(vc + 3 * i0), c1 = ld3(vc + 3 * i1), c2 = ld3(vc + 3 * i2);
     alb = c0 * abc.x + c1 * abc.y + c2 * abc.z;
     const V3 ga = v3(g.x * L.x, g.y * L.y, g.z * L.z);
 #pragma unroll
     for (int r = 0; r < 3; ++r) { t[G_COL + 3 * r] = ga.x * w3[r]; t[G_COL + 3 * r + 1] = ga.y * w3[r]; t[G_COL + 3 * r + 2] = ga.z * w3[r]; }
   } else if (ALBEDO == kTextured) {
-    // the backward's albedo is the reference's four-texel blend (Grad.cu:282-312), not the
-    // forward's nearest texel
-    const TexCoord tc = tex_coord(a.uv + 6 * (size_t)idf, abc, a.texW, a.texH);
-    const float* tb = a.tex + (size_t)b * a.texH * a.texW * 3;
     const float LU = tc.LU + 0.5f, HU = tc.LU + 1.5f, LV = tc.LV + 0.5f, HV = tc.LV + 1.5f;
-    const int lu = min(max(tc.LU, 0), a.texW - 1), hu = min(max(tc.LU + 1, 0), a.texW - 1);
-    const int lv = min(max(tc.LV, 0), a.texH - 1), hv = min(max(tc.LV + 1, 0), a.texH - 1);
-    const V3 cLL = ld3(tb + 3 * ((size_t)lv * a.texW + lu)), cHL = ld3(tb + 3 * ((size_t)lv * a.texW + hu));
-    const V3 cLH = ld3(tb + 3 * ((size_t)hv * a.texW + lu)), cHH = ld3(tb + 3 * ((size_t)hv * a.texW + hu));
     const float U0 = tc.U0, V0 = tc.V0;
-    alb = (V0 - LV) * (((U0 - LU) * cLL) + ((HU - U0) * cHL)) + (HV - V0) * (((U0 - LU) * cLH) + ((HU - U0) * cHH));
+    alb = blend4(U0 - LU, V0 - LV, HU - U0, HV - V0, cLL, cHL, cLH, cHH);
     if (!flipped && tc.LV >= 0 && tc.LV < a.texH && tc.LU >= 0 && tc.LU < a.texW) {
-      float* gt = a.g_tex + (size_t)b * a.texH * a.texW * 3 + 3 * ((size_t)tc.LV * a.texW + tc.LU);
-      atomicAdd(gt + 0, g.x * L.x);
-      atomicAdd(gt + 1, g.y * L.y);
-      atomicAdd(gt + 2, g.z * L.z);
+      tex_key = tc.LV * a.texW + tc.LU;
+      tex_val[0] = g.x * L.x;
+      tex_val[1] = g.y * L.y;
+      tex_val[2] = g.z * L.z;
     }
   }
 
   if (SHADED) {
+    // continuous terms only (the flip decision is taken): products and sums may fuse
+#pragma clang fp contract(fast)
     const V3 gl = v3(g.x * alb.x, g.y * alb.y, g.z * alb.z);
-    const float Y[9] = {1.f, n.y, n.z, n.x, n.x * n.y, n.z * n.y, 3.f * n.z * n.z - 1.f, n.x * n.z,
-                        n.x * n.x - n.y * n.y};
     const float glc[3] = {gl.x, gl.y, gl.z};
 #pragma unroll
     for (int ch = 0; ch < 3; ++ch)
@@ -198,7 +278,7 @@ __device__ __forceinline__ void pixel_terms(const BackwardArgs& a, int bc, int b
                      (gn[2] * unl2 - un.z * gnu) * inv_unl3);
 #pragma unroll
     for (int r = 0; r < 3; ++r) { t[G_VN + 3 * r] = gu.x * w3[r]; t[G_VN + 3 * r + 1] = gu.y * w3[r]; t[G_VN + 3 * r + 2] = gu.z * w3[r]; }
-    const V3 gb = v3(dot(gu, n0), dot(gu, n1), dot(gu, n2));
+    const V3 gb = v3(dot(f3(gu), f3(n0)), dot(f3(gu), f3(n1)), dot(f3(gu), f3(n2)));
     bary_position_grad<G_POS>(o, d, p0, p1, p2, gb, t);
   }
 
@@ -281,14 +361,38 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
   const int lane = threadIdx.x;
   const int tiles = a.tiles_x * a.tiles_y;
   const int64_t work = xcd_contiguous(blockIdx.x, gridDim.x);    // one (view, tile) per workgroup
-  const int tile = (int)(work % tiles), bc = (int)(work / tiles);
-  const int b = bc / a.C;
+  const int tile = __builtin_amdgcn_readfirstlane((int)(work % tiles));
+  const int bc = __builtin_amdgcn_readfirstlane((int)(work / tiles));
+  const int b = __builtin_amdgcn_readfirstlane(bc / a.C);
   const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
+  // the view's camera and SH coefficients: uniform, loaded once before any store of the kernel
+  // (so into scalar registers), not per pixel
+  Camera cam;
+  {
+    const Camera& src = a.cams[bc];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) cam.invP[k] = src.invP[k];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) { cam.o[k] = src.o[k]; cam.os[k] = src.os[k]; }
+    if (M2D) {
+#pragma unroll
+      for (int k = 0; k < 12; ++k) cam.E[k] = src.E[k];
+#pragma unroll
+      for (int k = 0; k < 9; ++k) cam.K[k] = src.K[k];
+    }
+  }
+  float shv[27];
+#pragma unroll
+  for (int k = 0; k < 27; ++k) shv[k] = SHADED ? a.sh[(size_t)bc * 27 + k] : 0.f;
   float sh_acc[27];
 #pragma unroll
   for (int k = 0; k < 27; ++k) sh_acc[k] = 0.f;
   bool any = false;
   int nslots = 0;
+  // the face of every slot, also held in registers for the lookups: lane j holds slot j in
+  // slot_face0 and slot 64 + j in slot_face1 (-1: free)
+  int slot_face0 = -1, slot_face1 = -1;
+  static_assert(kFaceSlots <= 128, "two registers of slot faces per lane");
 
   auto flush = [&]() {
     __syncthreads();
@@ -303,6 +407,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
     }
     __syncthreads();
     nslots = 0;
+    slot_face0 = slot_face1 = -1;
   };
 
   // Software pipeline over the rounds: the next round's face id, and then its vertex ids, are
@@ -331,10 +436,27 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
     float t[NT1 + 9];     // +9: unused groups of pixel_terms write past NT1 only in dead code
 #pragma unroll
     for (int k = 0; k < NT1 + 9; ++k) t[k] = 0.f;
+    int tex_key = -1;
+    float tex_val[3] = {0.f, 0.f, 0.f};
     if (idf >= 0)
-      pixel_terms<ALBEDO, SHADED, M2D, HAVE_BARY, G_POS, G_VN, G_COL>(a, bc, b, px, py, idx, idf, i0, i1, i2, t, sh_acc);
+      pixel_terms<ALBEDO, SHADED, M2D, HAVE_BARY, G_POS, G_VN, G_COL>(a, cam, shv, bc, b, px, py, idx, idf, i0, i1, i2, t,
+                                                                     sh_acc, tex_key, tex_val);
     if (round + 1 < RT * RT && nidf >= 0) {
       ni0 = a.faces[3 * nidf]; ni1 = a.faces[3 * nidf + 1]; ni2 = a.faces[3 * nidf + 2];
+    }
+    if (ALBEDO == kTextured && pending) {
+      // texture-gradient terms: lanes of one texel are merged first (about 1.8 pixels share a
+      // texel in a round at one texel per pixel), then one atomic per texel and channel
+      merge_pairs<0xB1, 0xB1, 1, 3>(tex_val, tex_key, lane);
+      merge_pairs<0x128, 0x128, 8, 3>(tex_val, tex_key, lane);
+      merge_pairs<0x4E, 0x4E, 2, 3>(tex_val, tex_key, lane);
+      merge_pairs<0x104, 0x114, 4, 3>(tex_val, tex_key, lane);
+      if (tex_key >= 0) {
+        float* gt = a.g_tex + ((size_t)b * a.texH * a.texW + tex_key) * 3;
+        atomicAdd(gt + 0, tex_val[0]);
+        atomicAdd(gt + 1, tex_val[1]);
+        atomicAdd(gt + 2, tex_val[2]);
+      }
     }
     if (!pending) continue;
     if (NT == 0) continue;
@@ -348,19 +470,35 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
       const int f = __builtin_amdgcn_readlane(idf, leader);
       const bool mine = idf == f;
       pending &= ~__ballot(mine);
-      int slot = -1;
-      for (int base = 0; base < nslots && slot < 0; base += 64) {
-        const int j = base + lane;
-        const uint64_t hit = __ballot(j < nslots && s_face[j] == f);
-        if (hit) slot = base + __ffsll((unsigned long long)hit) - 1;
-      }
-      if (slot < 0) {
-        slot = nslots++;
-        if (lane == 0) s_face[slot] = f;
-        for (int k = lane; k < NT; k += 64) s_acc[slot][k] = 0.f;
+      // lookup in the register copy of the slot table: compares and ballots, no LDS round trip
+      int slot;
+      const uint64_t hit0 = __ballot(slot_face0 == f);
+      if (hit0) {
+        slot = __ffsll((unsigned long long)hit0) - 1;
+      } else {
+        const uint64_t hit1 = __ballot(slot_face1 == f);
+        if (hit1) {
+          slot = 64 + __ffsll((unsigned long long)hit1) - 1;
+        } else {
+          slot = nslots++;
+          if (slot < 64) {
+            if (lane == slot) slot_face0 = f;
+          } else if (lane == slot - 64) {
+            slot_face1 = f;
+          }
+          if (lane == 0) s_face[slot] = f;
+          for (int k = lane; k < NT; k += 64) s_acc[slot][k] = 0.f;
+        }
       }
       if (mine) myslot = slot;
     }
+    // lanes are 8x8 pixels (x = lane & 7, y = lane >> 3): merge x neighbours (quad_perm
+    // [1,0,3,2]), y neighbours (row_ror:8), x pairs two apart (quad_perm [2,3,0,1]), then
+    // four apart (row_shl:4 / row_shr:4); survivors hold the sums of up to 16 lanes
+    merge_pairs<0xB1, 0xB1, 1, NT>(t, myslot, lane);
+    merge_pairs<0x128, 0x128, 8, NT>(t, myslot, lane);
+    merge_pairs<0x4E, 0x4E, 2, NT>(t, myslot, lane);
+    merge_pairs<0x104, 0x114, 4, NT>(t, myslot, lane);
     __syncthreads();
     if (myslot >= 0) {
 #pragma unroll
