@@ -358,6 +358,9 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
   constexpr int NT1 = NT > 0 ? NT : 1;
   __shared__ float s_acc[kFaceSlots][NT1];
   __shared__ int s_face[kFaceSlots];
+  constexpr int TS = ALBEDO == kTextured ? 64 : 1;
+  __shared__ int s_tkey[TS];          // a round's texture-gradient terms, compacted
+  __shared__ float s_tval[TS * 3];
   const int lane = threadIdx.x;
   const int tiles = a.tiles_x * a.tiles_y;
   const int64_t work = xcd_contiguous(blockIdx.x, gridDim.x);    // one (view, tile) per workgroup
@@ -451,12 +454,25 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
       merge_pairs<0x128, 0x128, 8, 3>(tex_val, tex_key, lane);
       merge_pairs<0x4E, 0x4E, 2, 3>(tex_val, tex_key, lane);
       merge_pairs<0x104, 0x114, 4, 3>(tex_val, tex_key, lane);
+      // the surviving texels are compacted into LDS and added with lane q taking channel q % 3
+      // of texel q / 3, so the three channels of a texel (and neighbouring texels) are adjacent
+      // addresses of one atomic instruction, not one address each of three
+      const uint64_t sm = __ballot(tex_key >= 0);
+      const int nsurv = __popcll(sm);
       if (tex_key >= 0) {
-        float* gt = a.g_tex + ((size_t)b * a.texH * a.texW + tex_key) * 3;
-        atomicAdd(gt + 0, tex_val[0]);
-        atomicAdd(gt + 1, tex_val[1]);
-        atomicAdd(gt + 2, tex_val[2]);
+        const int r = __builtin_amdgcn_mbcnt_hi((unsigned)(sm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)sm, 0u));
+        s_tkey[r] = tex_key;
+        s_tval[3 * r + 0] = tex_val[0];
+        s_tval[3 * r + 1] = tex_val[1];
+        s_tval[3 * r + 2] = tex_val[2];
       }
+      __syncthreads();
+      float* gtb = a.g_tex + (size_t)b * a.texH * a.texW * 3;
+      for (int q = lane; q < 3 * nsurv; q += 64) {
+        const int sidx = q / 3;
+        atomicAdd(gtb + 3 * (size_t)s_tkey[sidx] + (q - 3 * sidx), s_tval[q]);
+      }
+      __syncthreads();
     }
     if (!pending) continue;
     if (NT == 0) continue;
