@@ -49,24 +49,61 @@ __global__ void __launch_bounds__(256) k_vertices(const float* __restrict__ vpos
                                                   const int32_t* __restrict__ csr_list, const Camera* __restrict__ cams,
                                                   float* __restrict__ vnormal, float4* __restrict__ proj,
                                                   int B, int C, int N, int F) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (int64_t)B * C * N) return;
-  int v = (int)(i % N);
-  int bc = (int)(i / N);
-  int b = bc / C;
+  // one thread per (view, vertex)
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = i < (int64_t)B * C * N;
+  const int v = valid ? (int)(i % N) : 0;
+  const int bc = valid ? (int)(i / N) : 0;
+  const int b = bc / C;
   const float* fn = fnormal + (size_t)b * F * 3;
-  int s = csr_start[v], n = csr_count[v];
+  const int s = valid ? csr_start[v] : 0, n = valid ? csr_count[v] : 0;
+  // Vertices of up to kHeavy incident faces sum them in their own lane, loading the face ids
+  // and then the normals of kBatch faces at a time.  More incident faces (the centre of a fan:
+  // hundreds) would make one lane's chain of loads the whole kernel's time, so the wavefront
+  // loads their normals 64 at a time and folds them in list order with readlane.  Both keep
+  // the reference's summation order.
+  constexpr int kBatch = 8, kHeavy = 2 * kBatch;
   V3 acc = v3(0.f, 0.f, 0.f);
-  if (n > 0) {
-    acc = ld3(fn + 3 * csr_list[s]);
-    for (int k = 1; k < n; ++k) {
-      V3 q = ld3(fn + 3 * csr_list[s + k]);
-      acc.x = acc.x + q.x; acc.y = acc.y + q.y; acc.z = acc.z + q.z;
+  if (n <= kHeavy) {
+    for (int k0 = 0; k0 < n; k0 += kBatch) {
+      int id[kBatch];
+#pragma unroll
+      for (int j = 0; j < kBatch; ++j) id[j] = k0 + j < n ? csr_list[s + k0 + j] : 0;
+      V3 q[kBatch];
+#pragma unroll
+      for (int j = 0; j < kBatch; ++j) q[j] = k0 + j < n ? ld3(fn + 3 * id[j]) : v3(0.f, 0.f, 0.f);
+#pragma unroll
+      for (int j = 0; j < kBatch; ++j) {
+        if (k0 + j >= n) break;
+        if (k0 + j == 0) acc = q[j];
+        else { acc.x = acc.x + q[j].x; acc.y = acc.y + q[j].y; acc.z = acc.z + q[j].z; }
+      }
     }
   }
+  const int lane = threadIdx.x & 63;
+  for (uint64_t heavy = __ballot(n > kHeavy); heavy; heavy &= heavy - 1) {
+    const int j = __ffsll((unsigned long long)heavy) - 1;
+    const int sj = __builtin_amdgcn_readlane(s, j), nj = __builtin_amdgcn_readlane(n, j);
+    const float* fnj = fnormal + (size_t)__builtin_amdgcn_readlane(b, j) * F * 3;
+    V3 sum = v3(0.f, 0.f, 0.f);
+    for (int k0 = 0; k0 < nj; k0 += 64) {
+      const int k = k0 + lane;
+      const V3 q = k < nj ? ld3(fnj + 3 * csr_list[sj + k]) : v3(0.f, 0.f, 0.f);
+      const int cnt = min(64, nj - k0);
+      for (int t = 0; t < cnt; ++t) {
+        const float qx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(q.x), t));
+        const float qy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(q.y), t));
+        const float qz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(q.z), t));
+        if (k0 + t == 0) sum = v3(qx, qy, qz);
+        else { sum.x = sum.x + qx; sum.y = sum.y + qy; sum.z = sum.z + qz; }
+      }
+    }
+    if (lane == j) acc = sum;
+  }
+  if (!valid) return;
   float* o = vnormal + 3 * i;
   o[0] = acc.x; o[1] = acc.y; o[2] = acc.z;
-  V3 p = project(cams[bc], ld3(vpos + ((size_t)b * N + v) * 3));
+  const V3 p = project(cams[bc], ld3(vpos + ((size_t)b * N + v) * 3));
   proj[i] = make_float4(p.x, p.y, p.z, 0.f);
 }
 
