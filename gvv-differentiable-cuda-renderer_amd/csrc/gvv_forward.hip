@@ -217,7 +217,7 @@ __global__ void __launch_bounds__(64) k_raster(const TriSetup* __restrict__ tris
   const bool in_range = i < (int64_t)B * C * F;
   const int f = in_range ? (int)(i % F) : 0;
   const int bc = in_range ? (int)(i / F) : 0;
-  bool small = false;
+  bool small = false, to_queue = false;
   EdgeTest et;
   et.on = false;
   int bw = 0, bh = 0, fx0 = 0, fy0 = 0;
@@ -227,8 +227,7 @@ __global__ void __launch_bounds__(64) k_raster(const TriSetup* __restrict__ tris
       bw = r.bx.x1 - r.bx.x0 + 1;
       bh = r.bx.y1 - r.bx.y0 + 1;
       if ((int64_t)bw * bh > large_box) {
-        const unsigned int slot = atomicAdd(large_count, 1u);
-        large[slot] = make_int4(bc, f, mesh_id, 0);
+        to_queue = true;
       } else {
         small = true;
         FaceStage s;
@@ -240,6 +239,18 @@ __global__ void __launch_bounds__(64) k_raster(const TriSetup* __restrict__ tris
         fx0 = r.bx.x0;
         fy0 = r.bx.y0;
       }
+    }
+  }
+  // large faces go to the queue with one atomic per wavefront (the cursor is a single word
+  // that every wavefront of the launch would otherwise hit once per face)
+  const uint64_t qm = __ballot(to_queue);
+  if (qm) {
+    unsigned int base = 0;
+    if (lane == __ffsll((unsigned long long)qm) - 1) base = atomicAdd(large_count, (unsigned)__popcll(qm));
+    base = __builtin_amdgcn_readlane(base, __ffsll((unsigned long long)qm) - 1);
+    if (to_queue) {
+      const unsigned off = __builtin_amdgcn_mbcnt_hi((unsigned)(qm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)qm, 0u));
+      large[base + off] = make_int4(bc, f, mesh_id, 0);
     }
   }
   if (!__ballot(small)) return;
@@ -274,7 +285,7 @@ __global__ void __launch_bounds__(64) k_raster(const TriSetup* __restrict__ tris
     const int j = __ffsll((unsigned long long)todo) - 1;
     todo &= todo - 1;
     const EdgeTest ej = readlane_edges(et, j);
-    const int x0 = __builtin_amdgcn_readlane(st[j].x0, 0), y0 = __builtin_amdgcn_readlane(st[j].y0, 0);
+    const int x0 = __builtin_amdgcn_readlane(fx0, j), y0 = __builtin_amdgcn_readlane(fy0, j);
     const int jbw = __builtin_amdgcn_readlane(bw, j), jbh = __builtin_amdgcn_readlane(bh, j);
     collect_face(ej, j, x0, y0, jbw, jbh, st, cand, count, cams, one_view, bc0, keys, W, H);
   }
@@ -356,9 +367,11 @@ __device__ __forceinline__ void raster_large_face(const TriSetup* __restrict__ t
   flush_candidates(st, cand, count, cams, true, bc, keys, W, H);
 }
 
-// Persistent wavefronts drain the large-face queue: each takes one face at a time from an
-// atomic cursor and rasterises it (raster_large_face).  Every wavefront leaves when the
-// cursor passes the queue length written by k_raster.
+// Persistent wavefronts drain the large-face queue: wavefront w rasterises faces w, w + G,
+// w + 2G, ... (G wavefronts) up to the queue length written by k_raster (raster_large_face).
+// A shared atomic cursor would balance better, but every wavefront taking every item from one
+// word serialises on that word's memory-side atomics: 4k cursor atomics took longer than the
+// faces themselves.
 __global__ void __launch_bounds__(64) k_raster_large(const TriSetup* __restrict__ tris, const int32_t* __restrict__ faces,
                                                      const float4* __restrict__ proj, const Camera* __restrict__ cams,
                                                      unsigned long long* __restrict__ keys,
@@ -368,14 +381,9 @@ __global__ void __launch_bounds__(64) k_raster_large(const TriSetup* __restrict_
   __shared__ unsigned cand[kCandCap];
   __shared__ int s_xl[64], s_start[64];
   const unsigned int n_large = counters[0];
-  for (;;) {
-    unsigned int item = 0;
-    if (threadIdx.x == 0) item = atomicAdd(&counters[1], 1u);
-    item = __builtin_amdgcn_readfirstlane(item);
-    if (item >= n_large) break;
+  for (unsigned int item = blockIdx.x; item < n_large; item += gridDim.x)
     raster_large_face(tris, faces, proj, cams, keys, C, N, F, W, H, large[item].x, large[item].y, st, cand, s_xl,
                       s_start);
-  }
 }
 
 // The same over a queue shared by several meshes: each entry names its mesh, whose raster
@@ -387,11 +395,7 @@ __global__ void __launch_bounds__(64) k_raster_large_multi(const RasterMesh* __r
   __shared__ unsigned cand[kCandCap];
   __shared__ int s_xl[64], s_start[64];
   const unsigned int n_large = counters[0];
-  for (;;) {
-    unsigned int item = 0;
-    if (threadIdx.x == 0) item = atomicAdd(&counters[1], 1u);
-    item = __builtin_amdgcn_readfirstlane(item);
-    if (item >= n_large) break;
+  for (unsigned int item = blockIdx.x; item < n_large; item += gridDim.x) {
     const int4 e = large[item];
     const RasterMesh m = meshes[e.z];
     raster_large_face(m.tris, m.faces, m.proj, m.cams, m.keys, m.C, m.N, m.F, m.W, m.H, e.x, e.y, st, cand, s_xl,

@@ -124,7 +124,7 @@ struct ForwardPlan {
   float4* proj;                      // [B][C][N]
   float* vnormal_out;                // [B][C][N][3]
   float* normal_map_out;             // [B][texH][texW][3]
-  unsigned int* counters;            // [2]: large-face queue length, drain cursor
+  unsigned int* counters;            // [2]: large-face queue length, (unused)
   int4* large;                       // large-face queue of (b*C + c, face, mesh, 0)
   int mesh_id;                       // written into the queue entries
   bool shared_queue;                 // the queue is shared by several meshes: the caller clears

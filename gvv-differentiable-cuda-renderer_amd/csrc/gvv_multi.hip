@@ -60,7 +60,7 @@ struct Multi {
   hipEvent_t fork[2] = {};
   hipEvent_t join[2][kLanes] = {};
   // the shared large-face queue and the raster state of every mesh it names
-  DevBuf<unsigned int> counters;                // [2]: queue length, drain cursor
+  DevBuf<unsigned int> counters;                // [2]: queue length, (unused)
   DevBuf<int4> large;                           // sum over meshes of B*C*F entries
   DevBuf<RasterMesh> table;                     // [meshes]
   std::vector<RasterMesh> host_table;           // what `table` holds

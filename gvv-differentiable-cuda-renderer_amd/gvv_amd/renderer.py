@@ -49,6 +49,7 @@ class Rasterizer:
         self.image_filter_size, self.texture_filter_size = int(image_filter_size), int(texture_filter_size)
         self.compute_normal_map = bool(compute_normal_map)
         self._h = None
+        self._zero = {}
         L = _lib.load()
         h = ctypes.c_void_p()
         rc = L.gvv_create(faces.ctypes.data_as(ctypes.c_void_p), faces.size, texcoords.ctypes.data_as(ctypes.c_void_p),
@@ -110,7 +111,12 @@ class Rasterizer:
         if self.compute_normal_map:
             nmap = torch.empty((B, texH, texW, 3), device=dev, dtype=torch.float32)
         else:
-            nmap = torch.zeros((), device=dev, dtype=torch.float32).expand(B, texH, texW, 3)
+            # one zero scalar per device, made on first use: no fill per call (which a stream
+            # capture of the call would otherwise record)
+            z = self._zero.get(dev)
+            if z is None:
+                z = self._zero[dev] = torch.zeros((), device=dev, dtype=torch.float32)
+            nmap = z.expand(B, texH, texW, 3)
         rc = _lib.load().gvv_forward(self._h, B, texH, texW, _ptr(vpos), _ptr(vcol), _ptr(tex), _ptr(sh), _ptr(extr),
                                      _ptr(intr), _ptr(bary), _ptr(face), _ptr(render), _ptr(vnormal),
                                      _ptr(nmap) if self.compute_normal_map else None, _stream_ptr(dev))
