@@ -554,24 +554,44 @@ __global__ void __launch_bounds__(256) k_sh_reduce(const float* __restrict__ par
 __global__ void __launch_bounds__(256) k_face_jac(const float* __restrict__ vpos, const int32_t* __restrict__ faces,
                                                   const float* __restrict__ g_vn, float* __restrict__ g_pos,
                                                   int B, int N, int F) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (int64_t)B * F) return;
-  int b = (int)(i / F), f = (int)(i % F);
-  int a0 = faces[3 * f], a1 = faces[3 * f + 1], a2 = faces[3 * f + 2];
-  const float* gv = g_vn + (size_t)b * N * 3;
-  V3 G = ld3(gv + 3 * a0);
-  if (a1 != a0) G = G + ld3(gv + 3 * a1);
-  if (a2 != a0 && a2 != a1) G = G + ld3(gv + 3 * a2);
-  if (G.x == 0.f && G.y == 0.f && G.z == 0.f) return;
-  const float* vp = vpos + (size_t)b * N * 3;
-  V3 vi = ld3(vp + 3 * a0), vj = ld3(vp + 3 * a1), vk = ld3(vp + 3 * a2);
-  V3 gk = cross(G, vj - vi);
-  V3 gj = cross(vk - vi, G);
-  V3 gi = -(gj + gk);
-  float* gp = g_pos + (size_t)b * N * 3;
-  atomicAdd(gp + 3 * a0 + 0, gi.x); atomicAdd(gp + 3 * a0 + 1, gi.y); atomicAdd(gp + 3 * a0 + 2, gi.z);
-  atomicAdd(gp + 3 * a1 + 0, gj.x); atomicAdd(gp + 3 * a1 + 1, gj.y); atomicAdd(gp + 3 * a1 + 2, gj.z);
-  atomicAdd(gp + 3 * a2 + 0, gk.x); atomicAdd(gp + 3 * a2 + 1, gk.y); atomicAdd(gp + 3 * a2 + 2, gk.z);
+  // the workgroup's 256 faces x 3 corners x 3 channels are staged in LDS and added with lane q
+  // taking channel q mod 3 of corner q / 3, so that a vertex's three channels are adjacent
+  // lanes of one atomic instruction (atomics cost per cache line touched, not per lane)
+  __shared__ int s_vtx[256 * 3];     // corner vertex ids (-1: nothing to add)
+  __shared__ int s_b[256];            // batch entry of the face
+  __shared__ float s_val[256 * 9];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = threadIdx.x;
+  s_vtx[3 * t] = s_vtx[3 * t + 1] = s_vtx[3 * t + 2] = -1;
+  if (i < (int64_t)B * F) {
+    const int b = (int)(i / F);
+    s_b[t] = b;
+    const int f = (int)(i % F);
+    const int a0 = faces[3 * f], a1 = faces[3 * f + 1], a2 = faces[3 * f + 2];
+    const float* gv = g_vn + (size_t)b * N * 3;
+    V3 G = ld3(gv + 3 * a0);
+    if (a1 != a0) G = G + ld3(gv + 3 * a1);
+    if (a2 != a0 && a2 != a1) G = G + ld3(gv + 3 * a2);
+    if (!(G.x == 0.f && G.y == 0.f && G.z == 0.f)) {
+      const float* vp = vpos + (size_t)b * N * 3;
+      const V3 vi = ld3(vp + 3 * a0), vj = ld3(vp + 3 * a1), vk = ld3(vp + 3 * a2);
+      const V3 gk = cross(G, vj - vi);
+      const V3 gj = cross(vk - vi, G);
+      const V3 gi = -(gj + gk);
+      s_vtx[3 * t] = a0; s_vtx[3 * t + 1] = a1; s_vtx[3 * t + 2] = a2;
+      float* v = s_val + 9 * t;
+      v[0] = gi.x; v[1] = gi.y; v[2] = gi.z;
+      v[3] = gj.x; v[4] = gj.y; v[5] = gj.z;
+      v[6] = gk.x; v[7] = gk.y; v[8] = gk.z;
+    }
+  }
+  __syncthreads();
+  for (int q = t; q < 256 * 9; q += 256) {
+    const int slot = q / 9, r = q - 9 * slot;
+    const int vtx = s_vtx[3 * slot + r / 3];
+    if (vtx < 0) continue;
+    atomicAdd(g_pos + ((size_t)s_b[slot] * N + vtx) * 3 + r % 3, s_val[q]);
+  }
 }
 
 template <int ALBEDO, bool SHADED, bool M2D, bool HAVE_BARY>
