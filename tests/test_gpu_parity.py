@@ -82,3 +82,11 @@ def test_tuning_settings(device, tuning):
     s.uv = np.random.Generator(np.random.PCG64(4)).uniform(0, 1, size=s.uv.shape).astype(np.float32)
     _run(s, device, "vertexColor", "shaded", m2d=True, tuning=tuning)
     _run(s, device, "textured", "shaded", m2d=False, tuning=tuning)
+
+
+def test_large_face_queue_longer_than_the_persistent_grid(device):
+    """More queued large faces than the large-face kernel's persistent wavefronts (4096): with a
+    16 px threshold nearly every face of this soup is queued (2 views x 5000 faces), so every
+    wavefront takes several in its fixed stride."""
+    s = scenes.random_soup(11, n_faces=5000, n_cameras=2, width=96, height=96)
+    _run(s, device, "vertexColor", "shaded", m2d=False, tuning=(16, 16))
