@@ -90,3 +90,21 @@ def test_large_face_queue_longer_than_the_persistent_grid(device):
     wavefront takes several in its fixed stride."""
     s = scenes.random_soup(11, n_faces=5000, n_cameras=2, width=96, height=96)
     _run(s, device, "vertexColor", "shaded", m2d=False, tuning=(16, 16))
+
+
+def test_nothing_visible(device):
+    """A mesh far outside every view: no pixel is covered, the buffers hold the background and
+    every gradient is zero, as in the oracle."""
+    s = scenes.random_soup(12, n_faces=60, n_cameras=2, width=40, height=24)
+    s.vpos = s.vpos + np.float32(5.0e6)
+    for albedo in ("vertexColor", "textured"):
+        _run(s, device, albedo, "shaded", m2d=True)
+
+
+@pytest.mark.parametrize("hw", [(1, 1), (3, 5), (17, 9)])
+def test_tiny_and_ragged_images(device, hw):
+    """Images smaller than one backward tile and not a multiple of any tile edge."""
+    h, w = hw
+    s = scenes.random_soup(13, n_faces=50, n_cameras=2, width=w, height=h)
+    for tuning in ((8, 64), (32, 16)):
+        _run(s, device, "textured", "shaded", m2d=True, tuning=tuning)
