@@ -476,10 +476,16 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
     }
     if (!pending) continue;
     if (NT == 0) continue;
-    // assign each distinct face of this round a slot (reusing slots of earlier rounds); a
-    // round adds at most 64 faces, so flushing first whenever fewer remain keeps every slot
-    // of this round valid until its terms are added
-    if (nslots > kFaceSlots - 64) flush();
+    // assign each distinct face of this round a slot (reusing slots of earlier rounds).  The
+    // round adds at most as many faces as it has runs of equal ids along the lanes (a run
+    // starts where a lane's face differs from the previous lane's, or at a 16-lane row start),
+    // so flushing first when fewer slots remain keeps every slot of this round valid until its
+    // terms are added, without flushing a tile whose faces all fit
+    {
+      const int prev = __builtin_amdgcn_update_dpp(-2, idf, 0x111, 0xf, 0xf, false);   // row_shr:1
+      const int runs = __popcll(__ballot(idf >= 0 && idf != prev));
+      if (nslots + runs > kFaceSlots) flush();
+    }
     int myslot = -1;
     while (pending) {
       const int leader = __ffsll((unsigned long long)pending) - 1;
