@@ -81,21 +81,37 @@ __global__ void __launch_bounds__(256) k_vertices(const float* __restrict__ vpos
     }
   }
   const int lane = threadIdx.x & 63;
+  __shared__ float s_fold[4][64 * 3];   // one row per wavefront of the 256-thread workgroup
+  float* fold = s_fold[threadIdx.x >> 6];
   for (uint64_t heavy = __ballot(n > kHeavy); heavy; heavy &= heavy - 1) {
     const int j = __ffsll((unsigned long long)heavy) - 1;
     const int sj = __builtin_amdgcn_readlane(s, j), nj = __builtin_amdgcn_readlane(n, j);
     const float* fnj = fnormal + (size_t)__builtin_amdgcn_readlane(b, j) * F * 3;
     V3 sum = v3(0.f, 0.f, 0.f);
-    for (int k0 = 0; k0 < nj; k0 += 64) {
-      const int k = k0 + lane;
-      const V3 q = k < nj ? ld3(fnj + 3 * csr_list[sj + k]) : v3(0.f, 0.f, 0.f);
-      const int cnt = min(64, nj - k0);
-      for (int t = 0; t < cnt; ++t) {
-        const float qx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(q.x), t));
-        const float qy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(q.y), t));
-        const float qz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(q.z), t));
-        if (k0 + t == 0) sum = v3(qx, qy, qz);
-        else { sum.x = sum.x + qx; sum.y = sum.y + qy; sum.z = sum.z + qz; }
+    // 256 entries per pass: the face ids of all four 64-entry chunks, then their normals, are
+    // loaded at once (entries past the end repeat the last and are not summed), then folded
+    for (int k0 = 0; k0 < nj; k0 += 256) {
+      int id[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) id[c] = csr_list[sj + min(k0 + 64 * c + lane, nj - 1)];
+      V3 q[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) q[c] = ld3(fnj + 3 * id[c]);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int cnt = min(64, nj - (k0 + 64 * c));
+        if (cnt <= 0) break;
+        // the chunk goes through this wavefront's LDS row and every lane folds it in list
+        // order from there (uniform-address reads pipeline; a readlane per value does not)
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        fold[3 * lane + 0] = q[c].x; fold[3 * lane + 1] = q[c].y; fold[3 * lane + 2] = q[c].z;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll 8
+        for (int t = 0; t < cnt; ++t) {
+          const float qx = fold[3 * t], qy = fold[3 * t + 1], qz = fold[3 * t + 2];
+          if (k0 + 64 * c + t == 0) sum = v3(qx, qy, qz);
+          else { sum.x = sum.x + qx; sum.y = sum.y + qy; sum.z = sum.z + qz; }
+        }
       }
     }
     if (lane == j) acc = sum;
