@@ -622,18 +622,18 @@ static inline unsigned blocks_for(int64_t n, int t = 256) { return (unsigned)((n
 hipError_t launch_backward(const BackwardPlan& p, hipStream_t s) {
   const BackwardArgs& a = p.a;
   {
-    ProfScope ps(p.prof, "k_cameras", s);
-    hipError_t e = launch_cameras(p.extr, p.intr, p.cams, a.B * a.C, s);
-    if (e != hipSuccess) return e;
-  }
-  {
-    ProfScope ps(p.prof, "k_fill_grads", s);
+    // cameras and the zero fills of the gradient outputs in one launch
+    ProfScope ps(p.prof, "k_setup", s);
     const int64_t nv = (int64_t)a.B * a.N * 3;
-    (void)launch_fill_f32(a.g_pos, nv, 0.f, s);
-    (void)launch_fill_f32(a.g_col, nv, 0.f, s);
-    (void)launch_fill_f32(a.g_tex, (int64_t)a.B * a.texH * a.texW * 3, 0.f, s);
-    if (p.shaded) (void)launch_fill_f32(a.g_vn, nv, 0.f, s);
-    else (void)launch_fill_f32(a.g_sh, (int64_t)a.B * a.C * 27, 0.f, s);
+    SetupArgs sa;
+    sa.extr = p.extr; sa.intr = p.intr; sa.cams = p.cams; sa.n_cams = a.B * a.C;
+    sa.fill[0] = a.g_pos; sa.fill_n[0] = nv;
+    sa.fill[1] = a.g_col; sa.fill_n[1] = nv;
+    sa.fill[2] = a.g_tex; sa.fill_n[2] = (int64_t)a.B * a.texH * a.texW * 3;
+    if (p.shaded) { sa.fill[3] = a.g_vn; sa.fill_n[3] = nv; }
+    else { sa.fill[3] = a.g_sh; sa.fill_n[3] = (int64_t)a.B * a.C * 27; }
+    hipError_t e = launch_setup(sa, s);
+    if (e != hipSuccess) return e;
   }
   {
     ProfScope ps(p.prof, "k_pixel_grad", s);

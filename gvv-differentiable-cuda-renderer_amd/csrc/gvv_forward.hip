@@ -1,45 +1,73 @@
 // gvv_forward.hip -- forward kernels of the rasterizer (gfx950).
 //
 // Replaces renderBuffersGPU (reference cpp/src/Renderer/CUDABasedRasterization.cu:449-473):
-//   k_cameras       per-view inverse matrices and ray origin           (.cu:23-67)
-//   k_face_setup    unnormalised world-space face normals and the scaled ray/triangle
-//                   constants, once per mesh and face rather than per view (.cu:122-141)
+//   k_setup         one launch for the short stages: per-view inverse matrices and ray origin
+//                   (.cu:23-67); unnormalised world-space face normals and the scaled
+//                   ray/triangle constants, once per mesh and face rather than per view
+//                   (.cu:122-141); zero fills (the backward's gradient buffers)
 //   k_vertices      vertex normals by CSR gather + per-view projection  (.cu:98-115,148-174)
 //   k_raster        per (view, face) bounding-box walk; 64-bit (depth key | face id)
 //                   atomicMin into the visibility buffer               (.cu:184-255)
 //   k_resolve       one thread per pixel: decode the winner, recompute its barycentrics,
 //                   shade, write all pixel outputs, reset the visibility key (.cu:74-91,262-408)
 //   k_normal_map    texel-space normal map                             (.cu:415-445)
+#include <algorithm>
+
 #include "gvv_kernels.hpp"
 #include "gvv_raster.hpp"
 
 namespace gvv {
 
-__global__ void __launch_bounds__(256) k_cameras(const float* __restrict__ extr, const float* __restrict__ intr,
-                                                 Camera* __restrict__ cams, int n) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  Camera c;
-  camera_setup(extr + 12 * (size_t)i, intr + 9 * (size_t)i, c);
-  cams[i] = c;
-}
+// The setup stages of one call as one launch (SetupArgs): blocks [0, cb) set up cameras (block
+// 0 also clears the large-face counters), blocks [cb, cb + fb) set up faces, and the rest
+// zero-fill the listed buffers, 16 bytes per lane where a buffer is 16-byte aligned.
+struct SetupGrid { SetupArgs a; int cam_blocks, face_blocks, fill_blocks; };
 
-// One thread per (b, face): the unnormalised world-space face normal (for the vertex normals)
-// and the scaled ray/triangle constants shared by every view and pixel of the face.
-__global__ void __launch_bounds__(256) k_face_setup(const float* __restrict__ vpos, const int32_t* __restrict__ faces,
-                                                    float* __restrict__ fnormal, TriSetup* __restrict__ tris,
-                                                    int B, int N, int F) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (int64_t)B * F) return;
-  int b = (int)(i / F), f = (int)(i % F);
-  const float* vp = vpos + (size_t)b * N * 3;
-  V3 v0 = ld3(vp + 3 * faces[3 * f]), v1 = ld3(vp + 3 * faces[3 * f + 1]), v2 = ld3(vp + 3 * faces[3 * f + 2]);
-  if (fnormal) {
-    V3 n = cross(v1 - v0, v2 - v0);
-    float* o = fnormal + 3 * i;
-    o[0] = n.x; o[1] = n.y; o[2] = n.z;
+__global__ void __launch_bounds__(256) k_setup(SetupGrid g) {
+  const SetupArgs& a = g.a;
+  const int t = threadIdx.x;
+  int blk = blockIdx.x;
+  if (blk < g.cam_blocks) {
+    if (blk == 0 && a.counters && t < 2) a.counters[t] = 0u;
+    const int i = blk * 256 + t;
+    if (i < a.n_cams) {
+      Camera c;
+      camera_setup(a.extr + 12 * (size_t)i, a.intr + 9 * (size_t)i, c);
+      a.cams[i] = c;
+    }
+    return;
   }
-  tris[i] = tri_setup(v0, v1, v2);
+  blk -= g.cam_blocks;
+  if (blk < g.face_blocks) {
+    const int64_t i = (int64_t)blk * 256 + t;
+    if (i >= (int64_t)a.B * a.F) return;
+    const int b = (int)(i / a.F), f = (int)(i % a.F);
+    const float* vp = a.vpos + (size_t)b * a.N * 3;
+    const V3 v0 = ld3(vp + 3 * a.faces[3 * f]), v1 = ld3(vp + 3 * a.faces[3 * f + 1]),
+             v2 = ld3(vp + 3 * a.faces[3 * f + 2]);
+    if (a.fnormal) {
+      const V3 n = cross(v1 - v0, v2 - v0);
+      float* o = a.fnormal + 3 * i;
+      o[0] = n.x; o[1] = n.y; o[2] = n.z;
+    }
+    a.tris[i] = tri_setup(v0, v1, v2);
+    return;
+  }
+  blk -= g.face_blocks;
+  const int64_t start = (int64_t)blk * 256 + t, stride = (int64_t)g.fill_blocks * 256;
+  for (int k = 0; k < kFillSegs; ++k) {
+    float* p = a.fill[k];
+    const int64_t n = a.fill_n[k];
+    if (!p || n <= 0) continue;
+    int64_t head = 0;
+    if (((uintptr_t)p & 15) == 0) {
+      const int64_t n4 = n >> 2;
+      float4* p4 = reinterpret_cast<float4*>(p);
+      for (int64_t i = start; i < n4; i += stride) p4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+      head = n4 << 2;
+    }
+    for (int64_t i = head + start; i < n; i += stride) p[i] = 0.f;
+  }
 }
 
 // One thread per (b, c, vertex): vertex normal as the CSR-ordered sum of incident face
@@ -519,14 +547,18 @@ hipError_t launch_fill_f32(float* p, int64_t n, float v, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_face_setup(const float* vpos, const int32_t* faces, float* fnormal, TriSetup* tris, int B, int N,
-                             int F, hipStream_t s) {
-  hipLaunchKernelGGL(k_face_setup, dim3(blocks_for((int64_t)B * F)), dim3(256), 0, s, vpos, faces, fnormal, tris, B, N, F);
-  return hipGetLastError();
-}
-
-hipError_t launch_cameras(const float* extr, const float* intr, Camera* cams, int n, hipStream_t s) {
-  hipLaunchKernelGGL(k_cameras, dim3(blocks_for(n)), dim3(256), 0, s, extr, intr, cams, n);
+hipError_t launch_setup(const SetupArgs& a, hipStream_t s) {
+  SetupGrid g;
+  g.a = a;
+  g.cam_blocks = a.n_cams > 0 || a.counters ? (int)blocks_for(a.n_cams > 0 ? a.n_cams : 1) : 0;
+  g.face_blocks = a.faces ? (int)blocks_for((int64_t)a.B * a.F) : 0;
+  int64_t most = 0;
+  for (int k = 0; k < kFillSegs; ++k)
+    if (a.fill[k] && a.fill_n[k] > most) most = a.fill_n[k];
+  g.fill_blocks = most > 0 ? (int)std::min<int64_t>(2048, (int64_t)blocks_for((most + 3) / 4)) : 0;
+  const int total = g.cam_blocks + g.face_blocks + g.fill_blocks;
+  if (total == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_setup, dim3(total), dim3(256), 0, s, g);
   return hipGetLastError();
 }
 
@@ -557,13 +589,14 @@ hipError_t launch_raster_large_multi(const RasterMesh* meshes, unsigned int* cou
 hipError_t launch_forward_raster(const ForwardPlan& p, hipStream_t s) {
   const int B = p.B, C = p.C, N = p.N, F = p.F;
   {
-    ProfScope ps(p.prof, "k_cameras", s);
-    hipError_t e = launch_cameras(p.extr, p.intr, p.cams, B * C, s);
-    if (e != hipSuccess) return e;
-  }
-  {
-    ProfScope ps(p.prof, "k_face_setup", s);
-    hipError_t e = launch_face_setup(p.vpos, p.faces, p.fnormal, p.tris, B, N, F, s);
+    // cameras, face setup and (for a queue of its own) the large-face counters
+    ProfScope ps(p.prof, "k_setup", s);
+    SetupArgs sa;
+    sa.extr = p.extr; sa.intr = p.intr; sa.cams = p.cams; sa.n_cams = B * C;
+    sa.vpos = p.vpos; sa.faces = p.faces; sa.fnormal = p.fnormal; sa.tris = p.tris;
+    sa.B = B; sa.N = N; sa.F = F;
+    sa.counters = p.shared_queue ? nullptr : p.counters;
+    hipError_t e = launch_setup(sa, s);
     if (e != hipSuccess) return e;
   }
   {
@@ -580,7 +613,6 @@ hipError_t launch_forward_raster(const ForwardPlan& p, hipStream_t s) {
   }
   {
     ProfScope ps(p.prof, "k_raster", s);
-    if (!p.shared_queue) hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, p.counters, 2);
     hipLaunchKernelGGL(k_raster, dim3(blocks_for((int64_t)B * C * F, 64)), dim3(64), 0, s, p.tris, p.faces, p.proj,
                        p.cams, p.r.keys, p.counters, p.large, B, C, N, F, p.r.W, p.r.H, p.large_box, p.mesh_id);
   }

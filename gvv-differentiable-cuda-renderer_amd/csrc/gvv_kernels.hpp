@@ -171,9 +171,26 @@ struct BackwardPlan {
 
 hipError_t launch_fill_u64(unsigned long long* p, int64_t n, unsigned long long v, hipStream_t s);
 hipError_t launch_fill_f32(float* p, int64_t n, float v, hipStream_t s);
-hipError_t launch_cameras(const float* extr, const float* intr, Camera* cams, int n, hipStream_t s);
-hipError_t launch_face_setup(const float* vpos, const int32_t* faces, float* fnormal, TriSetup* tris, int B, int N,
-                             int F, hipStream_t s);
+
+// The short setup stages of a call in one launch: per-view cameras, the per-face setup
+// (when faces is set), zero fills of up to kFillSegs float buffers, and the two large-face
+// counters (when set).
+constexpr int kFillSegs = 5;
+struct SetupArgs {
+  const float* extr = nullptr;
+  const float* intr = nullptr;
+  Camera* cams = nullptr;
+  int n_cams = 0;
+  const float* vpos = nullptr;
+  const int32_t* faces = nullptr;
+  float* fnormal = nullptr;
+  TriSetup* tris = nullptr;
+  int B = 0, N = 0, F = 0;
+  float* fill[kFillSegs] = {};
+  int64_t fill_n[kFillSegs] = {};
+  unsigned int* counters = nullptr;
+};
+hipError_t launch_setup(const SetupArgs& a, hipStream_t s);
 hipError_t launch_forward(const ForwardPlan& p, hipStream_t s);
 // The two halves of launch_forward: up to and including the depth pass (with the large-face
 // queue unless the plan's queue is shared), then resolve.
