@@ -358,7 +358,7 @@ def main():
             counts = json.load(fh)
         peak = atomic_peak(dev)
         atomics = {}
-        for k in ("k_pixel_grad", "k_raster", "k_raster_large", "k_face_jac"):
+        for k in ("k_pixel_grad", "k_raster", "k_raster_large", "k_grad_finish"):
             if k in counts and k in kernels:
                 rate = counts[k] / (kernels[k][0] / kernels[k][1] / 1e3)
                 atomics[k] = dict(requests=counts[k], rate_per_s=round(rate, 1), peak_per_s=round(peak["streaming"], 1),

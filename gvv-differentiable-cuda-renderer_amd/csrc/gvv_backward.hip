@@ -9,8 +9,8 @@
 //                  (one per distinct face of the tile) before lane-parallel global atomics
 //                  flush them to the face's three vertices; SH terms accumulate in registers
 //                  over the tile and are reduced once per tile (DPP) into a partial buffer.
-//   k_sh_reduce    fixed-order reduction of the SH partials (deterministic).
-//   k_face_jac     the normal -> position chain.  Instead of walking every incident face of
+//   k_grad_finish  one launch of two parts.  SH: fixed-order reduction of the SH partials
+//                  (deterministic).  Face Jacobian: the normal -> position chain.  Instead of walking every incident face of
 //                  every corner for every pixel (Grad.cu:559-615), the pixels accumulate
 //                  dL/d(vertex normal) per vertex; one thread per face then forms
 //                  G = sum of its vertices' normal gradients and scatters G x (vj - vi),
@@ -541,9 +541,9 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
 
 // Fixed-order reduction of the SH tile partials ([b*C + c][27][tiles]): one workgroup per
 // (b, c, coefficient), so the result does not depend on scheduling.
-__global__ void __launch_bounds__(256) k_sh_reduce(const float* __restrict__ partial, float* __restrict__ g_sh, int tiles) {
-  __shared__ float red[256];
-  const int bc = blockIdx.x, k = blockIdx.y;
+__device__ __forceinline__ void sh_reduce_block(const float* __restrict__ partial, float* __restrict__ g_sh, int tiles,
+                                                int blk, float* red) {
+  const int bc = blk / 27, k = blk % 27;
   const float* p = partial + ((size_t)bc * 27 + k) * tiles;
   float s = 0.f;
   for (int t = threadIdx.x; t < tiles; t += 256) s += p[t];
@@ -556,17 +556,18 @@ __global__ void __launch_bounds__(256) k_sh_reduce(const float* __restrict__ par
   if (threadIdx.x == 0) g_sh[(size_t)bc * 27 + k] = red[0];
 }
 
-// Normal -> position chain regrouped by face (see the file header).
-__global__ void __launch_bounds__(256) k_face_jac(const float* __restrict__ vpos, const int32_t* __restrict__ faces,
-                                                  const float* __restrict__ g_vn, float* __restrict__ g_pos,
-                                                  int B, int N, int F) {
+// Normal -> position chain regrouped by face (see the file header), for the workgroup's 256
+// faces.
+__device__ __forceinline__ void face_jac_block(const float* __restrict__ vpos, const int32_t* __restrict__ faces,
+                                               const float* __restrict__ g_vn, float* __restrict__ g_pos, int B, int N,
+                                               int F, int blk, float* smem) {
   // the workgroup's 256 faces x 3 corners x 3 channels are staged in LDS and added with lane q
   // taking channel q mod 3 of corner q / 3, so that a vertex's three channels are adjacent
   // lanes of one atomic instruction (atomics cost per cache line touched, not per lane)
-  __shared__ int s_vtx[256 * 3];     // corner vertex ids (-1: nothing to add)
-  __shared__ int s_b[256];            // batch entry of the face
-  __shared__ float s_val[256 * 9];
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int* s_vtx = reinterpret_cast<int*>(smem);            // [256 * 3] corner vertex ids (-1: nothing)
+  int* s_b = s_vtx + 256 * 3;                            // [256] batch entry of the face
+  float* s_val = reinterpret_cast<float*>(s_b + 256);    // [256 * 9]
+  const int64_t i = (int64_t)blk * 256 + threadIdx.x;
   const int t = threadIdx.x;
   s_vtx[3 * t] = s_vtx[3 * t + 1] = s_vtx[3 * t + 2] = -1;
   if (i < (int64_t)B * F) {
@@ -598,6 +599,17 @@ __global__ void __launch_bounds__(256) k_face_jac(const float* __restrict__ vpos
     if (vtx < 0) continue;
     atomicAdd(g_pos + ((size_t)s_b[slot] * N + vtx) * 3 + r % 3, s_val[q]);
   }
+}
+
+// The two reductions after the pixel kernel in one launch: blocks [0, fj_blocks) run the
+// face Jacobian, the rest the SH reduction (one per view and coefficient).
+__global__ void __launch_bounds__(256) k_grad_finish(const float* __restrict__ vpos, const int32_t* __restrict__ faces,
+                                                     const float* __restrict__ g_vn, float* __restrict__ g_pos,
+                                                     const float* __restrict__ sh_partial, float* __restrict__ g_sh,
+                                                     int B, int N, int F, int tiles, int fj_blocks) {
+  __shared__ float smem[256 * 13];
+  if ((int)blockIdx.x < fj_blocks) face_jac_block(vpos, faces, g_vn, g_pos, B, N, F, blockIdx.x, smem);
+  else sh_reduce_block(sh_partial, g_sh, tiles, blockIdx.x - fj_blocks, smem);
 }
 
 template <int ALBEDO, bool SHADED, bool M2D, bool HAVE_BARY>
@@ -647,13 +659,10 @@ hipError_t launch_backward(const BackwardPlan& p, hipStream_t s) {
     }
   }
   if (p.shaded) {
-    {
-      ProfScope ps(p.prof, "k_sh_reduce", s);
-      hipLaunchKernelGGL(k_sh_reduce, dim3(a.B * a.C, 27), dim3(256), 0, s, a.sh_partial, a.g_sh, a.tiles_x * a.tiles_y);
-    }
-    ProfScope ps(p.prof, "k_face_jac", s);
-    hipLaunchKernelGGL(k_face_jac, dim3(blocks_for((int64_t)a.B * a.F)), dim3(256), 0, s, a.vpos, a.faces, a.g_vn,
-                       a.g_pos, a.B, a.N, a.F);
+    ProfScope ps(p.prof, "k_grad_finish", s);
+    const int fj_blocks = (int)blocks_for((int64_t)a.B * a.F);
+    hipLaunchKernelGGL(k_grad_finish, dim3(fj_blocks + a.B * a.C * 27), dim3(256), 0, s, a.vpos, a.faces, a.g_vn,
+                       a.g_pos, a.sh_partial, a.g_sh, a.B, a.N, a.F, a.tiles_x * a.tiles_y, fj_blocks);
   }
   return hipGetLastError();
 }
