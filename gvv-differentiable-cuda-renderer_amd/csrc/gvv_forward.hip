@@ -176,22 +176,43 @@ struct FaceStage {
   int f, x0, y0, bw, bc;
 };
 
-// Candidate list of one wavefront: (face slot << 24 | pixel index inside the face's box).
+// Candidate list of one wavefront: 32-bit words naming a staged face (its slot) and a pixel of
+// its box.  Images up to 8192 px on a side pack (slot << 26 | dy << 13 | dx), so the exact test
+// needs no division; larger ones keep (slot << 24 | dy * bw + dx), where the depth pass's boxes
+// stay below 2^24 px (large_box <= 2^20) and the large-face pass, whose one face is slot 0,
+// gives all 32 bits to the pixel index.
 constexpr int kCandCap = 512;
+__device__ __forceinline__ bool cand_packed(int W, int H) { return W <= 8192 && H <= 8192; }
+__device__ __forceinline__ unsigned cand_word(bool packed, bool single, int slot, int dx, int dy, int bw) {
+  if (packed) return ((unsigned)slot << 26) | ((unsigned)dy << 13) | (unsigned)dx;
+  const unsigned p = (unsigned)dy * (unsigned)bw + (unsigned)dx;
+  return single ? p : (((unsigned)slot << 24) | p);
+}
 
 // Exact test of every staged candidate (reference uv2barycentric + inside test + depth key,
 // CUDABasedRasterization.cu:237-250), then the 64-bit minimum into the visibility buffer.
 // Each lane takes candidates lane, lane+64, ...; all lanes run the expensive test together.
+// `single`: the candidates of the large-face pass, which stages one face (slot 0).
 __device__ __forceinline__ void flush_candidates(const FaceStage* st, const unsigned* cand, int count,
                                                  const Camera* __restrict__ cams, bool one_view, int bc0,
-                                                 unsigned long long* __restrict__ keys, int W, int H) {
+                                                 unsigned long long* __restrict__ keys, int W, int H, bool single) {
   const int lane = threadIdx.x & 63;
+  const bool packed = cand_packed(W, H);
   for (int i = lane; i < count; i += 64) {
     const unsigned c = cand[i];
-    const FaceStage& fs = st[c >> 24];
-    const int p = (int)(c & 0xffffffu);
-    const int dy = p / fs.bw;
-    const int u = fs.x0 + (p - dy * fs.bw), v = fs.y0 + dy;
+    int slot, dx, dy;
+    if (packed) {
+      slot = single ? 0 : (int)(c >> 26);
+      dy = (int)((c >> 13) & 0x1fffu);
+      dx = (int)(c & 0x1fffu);
+    } else {
+      slot = single ? 0 : (int)(c >> 24);
+      const unsigned p = single ? c : (c & 0xffffffu), w = (unsigned)st[slot].bw;
+      dy = (int)(p / w);
+      dx = (int)(p - (unsigned)dy * w);
+    }
+    const FaceStage& fs = st[slot];
+    const int u = fs.x0 + dx, v = fs.y0 + dy;
     const Camera& cam = cams[one_view ? bc0 : fs.bc];
     const V3 d = pixel_ray(cam, u + 0.5f, v + 0.5f);
     const V3 abc = ray_barycentric(fs.t, cam_origin_scaled(cam), d);
@@ -211,20 +232,22 @@ __device__ __forceinline__ void collect_face(const EdgeTest& et_in, int slot, in
   for (int base = 0; base < area; base += 64) {
     if (count > kCandCap - 64) {
       __syncthreads();
-      flush_candidates(st, cand, count, cams, one_view, bc0, keys, W, H);
+      flush_candidates(st, cand, count, cams, one_view, bc0, keys, W, H, false);
       __syncthreads();
       count = 0;
     }
     const int p = base + lane;
     bool pass = false;
+    int dx = 0, dy = 0;
     if (p < area) {
-      const int dy = p / bw;
-      pass = et_in.pass(x0 + (p - dy * bw) + 0.5f, y0 + dy + 0.5f);
+      dy = p / bw;
+      dx = p - dy * bw;
+      pass = et_in.pass(x0 + dx + 0.5f, y0 + dy + 0.5f);
     }
     const uint64_t m = __ballot(pass);
     if (pass) {
       const int off = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
-      cand[count + off] = ((unsigned)slot << 24) | (unsigned)p;
+      cand[count + off] = cand_word(cand_packed(W, H), false, slot, dx, dy, bw);
     }
     count += __popcll(m);
   }
@@ -311,16 +334,17 @@ __global__ void __launch_bounds__(64) k_raster(const TriSetup* __restrict__ tris
   for (int p = 0; __ballot(p < my_area); ++p) {
     if (count > kCandCap - 64) {
       __syncthreads();
-      flush_candidates(st, cand, count, cams, one_view, bc0, keys, W, H);
+      flush_candidates(st, cand, count, cams, one_view, bc0, keys, W, H, false);
       __syncthreads();
       count = 0;
     }
     const bool pass = p < my_area && et.pass(fx0 + dx + 0.5f, fy0 + dy + 0.5f);
+    const unsigned word = cand_word(cand_packed(W, H), false, lane, dx, dy, bw);
     if (++dx == bw) { dx = 0; ++dy; }
     const uint64_t m = __ballot(pass);
     if (pass) {
       const int off = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
-      cand[count + off] = ((unsigned)lane << 24) | (unsigned)p;
+      cand[count + off] = word;
     }
     count += __popcll(m);
   }
@@ -335,7 +359,7 @@ __global__ void __launch_bounds__(64) k_raster(const TriSetup* __restrict__ tris
     collect_face(ej, j, x0, y0, jbw, jbh, st, cand, count, cams, one_view, bc0, keys, W, H);
   }
   __syncthreads();
-  flush_candidates(st, cand, count, cams, one_view, bc0, keys, W, H);
+  flush_candidates(st, cand, count, cams, one_view, bc0, keys, W, H, false);
 }
 
 // One large face (bc, f) rasterised by the whole wavefront.  Instead of the whole bounding box
@@ -382,7 +406,7 @@ __device__ __forceinline__ void raster_large_face(const TriSetup* __restrict__ t
     for (int q0 = 0; q0 < total; q0 += 64) {
       if (count > kCandCap - 64) {
         __syncthreads();
-        flush_candidates(st, cand, count, cams, true, bc, keys, W, H);
+        flush_candidates(st, cand, count, cams, true, bc, keys, W, H, true);
         __syncthreads();
         count = 0;
       }
@@ -398,7 +422,7 @@ __device__ __forceinline__ void raster_large_face(const TriSetup* __restrict__ t
         const int x = s_xl[row] + (q - s_start[row]);
         const int yy = r.bx.y0 + rbase + row;
         pass = et.pass(x + 0.5f, yy + 0.5f);
-        p = (yy - r.bx.y0) * bw + (x - r.bx.x0);
+        p = (int)cand_word(cand_packed(W, H), true, 0, x - r.bx.x0, yy - r.bx.y0, bw);
       }
       const uint64_t m = __ballot(pass);
       if (pass) {
@@ -409,7 +433,7 @@ __device__ __forceinline__ void raster_large_face(const TriSetup* __restrict__ t
     }
   }
   __syncthreads();
-  flush_candidates(st, cand, count, cams, true, bc, keys, W, H);
+  flush_candidates(st, cand, count, cams, true, bc, keys, W, H, true);
 }
 
 // Persistent wavefronts drain the large-face queue: wavefront w rasterises faces w, w + G,

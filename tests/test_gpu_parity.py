@@ -108,3 +108,17 @@ def test_tiny_and_ragged_images(device, hw):
     s = scenes.random_soup(13, n_faces=50, n_cameras=2, width=w, height=h)
     for tuning in ((8, 64), (32, 16)):
         _run(s, device, "textured", "shaded", m2d=True, tuning=tuning)
+
+
+def test_wide_images_unpacked_candidates(device):
+    """Images more than 8192 px wide keep the unpacked candidate words (slot | pixel index)."""
+    s = scenes.config1(width=8300, height=64, cols=120, rows=40, n_cameras=1, tex_size=64)
+    _run(s, device, "textured", "shaded", m2d=False)
+
+
+def test_face_box_beyond_24_bits(device):
+    """One face covering an 8300 x 2100 view: its box holds 17.4 M pixels, past 2^24, so the
+    large-face pass needs all 32 bits of the candidate word for the pixel index."""
+    s = scenes.single_triangle(width=8300, height=2100, z_m=0.8,
+                               tri_m=((-2.0, -1.0, 0.0), (2.0, -1.0, 0.0), (0.0, 1.2, 0.0)))
+    _run(s, device, "vertexColor", "shaded", m2d=False, check_grads=False)
