@@ -478,9 +478,17 @@ __global__ void k_zero_u32(unsigned int* p, int n) {
 
 template <int ALBEDO, bool SHADE>
 __global__ void __launch_bounds__(256) k_resolve(ResolveArgs a) {
-  int64_t i = xcd_contiguous(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  const int64_t P = (int64_t)a.W * a.H;
-  if (i >= (int64_t)a.B * a.C * P) return;
+  // one workgroup per 256-pixel chunk of one image row: the view, row and chunk follow from the
+  // workgroup index alone (uniform arithmetic), so no lane divides its pixel index
+  const unsigned chunks = ((unsigned)a.W + 255u) >> 8;
+  const unsigned per_view = chunks * (unsigned)a.H;
+  const unsigned wg = (unsigned)xcd_contiguous(blockIdx.x, gridDim.x);
+  const int bc = (int)(wg / per_view);
+  const unsigned rem = wg - (unsigned)bc * per_view;
+  const int py = (int)(rem / chunks);
+  const int px = (int)(rem - (unsigned)py * chunks) * 256 + (int)threadIdx.x;
+  if (bc >= a.B * a.C || px >= a.W) return;
+  const int64_t i = ((int64_t)bc * a.H + py) * a.W + px;
   unsigned long long key = a.keys[i];
   if (key == kEmptyKey) {
     a.face[i] = -1;
@@ -489,10 +497,7 @@ __global__ void __launch_bounds__(256) k_resolve(ResolveArgs a) {
     return;
   }
   a.keys[i] = kEmptyKey;
-  int bc = (int)(i / P);
-  int pix = (int)(i % P);
-  int b = bc / a.C, c = bc % a.C;
-  int py = pix / a.W, px = pix % a.W;
+  const int b = bc / a.C;
   int f = (int)(uint32_t)(key & 0xffffffffu);
   int i0 = a.faces[3 * f], i1 = a.faces[3 * f + 1], i2 = a.faces[3 * f + 2];
   // recomputed from the vertices: cheaper here than a 64-byte gather of the face's constants
@@ -589,8 +594,9 @@ hipError_t launch_setup(const SetupArgs& a, hipStream_t s) {
 
 template <int ALBEDO, bool SHADE>
 static void launch_resolve_t(const ResolveArgs& a, hipStream_t s) {
-  int64_t n = (int64_t)a.B * a.C * a.W * a.H;
-  hipLaunchKernelGGL((k_resolve<ALBEDO, SHADE>), dim3(blocks_for(n)), dim3(256), 0, s, a);
+  const int64_t n = (int64_t)a.B * a.C * a.H * ((a.W + 255) / 256);   // workgroups: row chunks
+  if (n <= 0) return;
+  hipLaunchKernelGGL((k_resolve<ALBEDO, SHADE>), dim3((unsigned)n), dim3(256), 0, s, a);
 }
 
 RasterMesh raster_mesh(const ForwardPlan& p) {
