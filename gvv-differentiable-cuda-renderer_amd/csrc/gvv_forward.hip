@@ -657,6 +657,8 @@ hipError_t launch_forward_raster(const ForwardPlan& p, hipStream_t s) {
 
 hipError_t launch_forward_resolve(const ForwardPlan& p, hipStream_t s) {
   // In normal-map mode the visibility buffer is empty, so this writes the background.
+  // One workgroup per 256-pixel row chunk: the count must fit a 32-bit grid dimension.
+  if ((int64_t)p.r.B * p.r.C * p.r.H * ((p.r.W + 255) / 256) > (int64_t)UINT32_MAX) return hipErrorInvalidConfiguration;
   ProfScope ps(p.prof, "k_resolve", s);
   switch (p.albedo * 2 + (p.shade ? 1 : 0)) {
     case kVertexColor * 2 + 0: launch_resolve_t<kVertexColor, false>(p.r, s); break;
