@@ -49,7 +49,7 @@ VALU_PEAK_PER_S = 256 * 4 * 2.4e9 / 4
 def kernel_bytes(name, scene, covered, texW, texH):
     """Algorithmic bytes of one launch of `name` for this workload: every distinct byte the
     kernel's algorithm must read or write once.  Per-unit figures:
-      k_pixel_grad  24 B/px (render grad 12 + face id 4 + barycentrics 8) + per vertex 12 (pos) + 12 C (normals)
+      k_pixel_grad  16 B/px (render grad 12 + face id 4) + per vertex 12 (pos) + 12 C (normals)
                     + 24 (pos and normal-gradient accumulators) + per face 36 (ids 12 + UVs 24)
                     + 24 B/texel (texture read 12 + texture-gradient write 12) + 108 B/view (SH)
       k_resolve     8 B/px key read + 24 B/px outputs (bary 8, face 4, render 12) + 8 B per
@@ -60,7 +60,7 @@ def kernel_bytes(name, scene, covered, texW, texH):
     P = B * C * scene.width * scene.height
     T = B * texW * texH
     if name == "k_pixel_grad":
-        return 24 * P + B * N * (12 + 12 * C + 24) + 36 * F + 24 * T + 108 * B * C
+        return 16 * P + B * N * (12 + 12 * C + 24) + 36 * F + 24 * T + 108 * B * C
     if name == "k_resolve":
         return 32 * P + 8 * covered + B * N * (12 + 12 * C) + 36 * F + 12 * T
     if name == "k_raster":
@@ -281,10 +281,8 @@ def main():
     covered = int((face >= 0).sum().item())
 
     def step():
-        # the backward takes the forward's barycentric and face buffers, as the reference's
-        # registered gradient does (python/CudaRenderer.py:190-191)
-        bb, f, _, vn, _ = rast.forward(vpos, vcol, tex, sh, extr, intr)
-        rast.backward(g_render, None, None, vpos, vcol, tex, sh, vn, f, extr, intr, bary=bb)
+        _, f, _, vn, _ = rast.forward(vpos, vcol, tex, sh, extr, intr)
+        rast.backward(g_render, None, None, vpos, vcol, tex, sh, vn, f, extr, intr)
 
     # launch settings: the fastest of the autotuner's candidates on this workload (untimed), or
     # the ones given with --tuning

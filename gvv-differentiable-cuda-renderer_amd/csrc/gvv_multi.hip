@@ -105,7 +105,7 @@ int plan_mesh(Multi* m, int i, const gvv_mesh_io& io, bool backward, hipStream_t
   if (backward) {
     if (!io.g_render) return fail(where + "g_render is null");
     if (plan_backward(h, io.batch, io.tex_h, io.tex_w, io.g_render, nullptr, nullptr, io.vpos, io.vcol, io.tex, io.sh,
-                      io.vnormal, io.bary, io.face, io.extr, io.intr, io.g_pos, io.g_col, io.g_tex, io.g_sh, bp))
+                      io.vnormal, nullptr, io.face, io.extr, io.intr, io.g_pos, io.g_col, io.g_tex, io.g_sh, bp))
       return fail(where + gvv_last_error());
   }
   return 0;

@@ -206,22 +206,19 @@ class RenderFunction(torch.autograd.Function):
         ctx.set_materialize_grads(False)
         bary, face, render, vnormal, nmap = rast.forward(vpos, vcol, tex, sh, extr, intr)
         ctx.rast = rast
-        # the registered gradient takes the forward's barycentric and face buffers
-        # (python/CudaRenderer.py:190-191), so the backward reads the barycentrics it needs
-        ctx.save_for_backward(vpos, vcol, tex, sh, target, extr, intr, vnormal, face, bary)
+        ctx.save_for_backward(vpos, vcol, tex, sh, target, extr, intr, vnormal, face)
         ctx.mark_non_differentiable(face)
         # target_image_out is the target input itself (the reference copies it, :286)
         return bary, face, render, vnormal, target, nmap
 
     @staticmethod
     def backward(ctx, g_bary, g_face, g_render, g_vnormal, g_target, g_nmap):
-        vpos, vcol, tex, sh, target, extr, intr, vnormal, face, bary = ctx.saved_tensors
+        vpos, vcol, tex, sh, target, extr, intr, vnormal, face = ctx.saved_tensors
         rast = ctx.rast
         if g_render is None and g_target is None:
             grads = (None, None, None, None)
         else:
-            grads = rast.backward(g_render, g_target, target, vpos, vcol, tex, sh, vnormal, face, extr, intr,
-                                  bary=bary)
+            grads = rast.backward(g_render, g_target, target, vpos, vcol, tex, sh, vnormal, face, extr, intr)
         need = ctx.needs_input_grad
         out = [None]
         for i, g in enumerate(grads):
