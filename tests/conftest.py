@@ -51,3 +51,20 @@ def elem_rel_stats(x, ref, floor=1e-3):
     m = np.abs(ref) > floor * np.abs(ref).max()
     r = np.abs(x[m] - ref[m]) / np.abs(ref[m])
     return float(r.max()), float(np.median(r))
+
+
+def pytest_terminal_summary(terminalreporter):
+    """Accuracy of every GPU-vs-oracle comparison at the full workload sizes (cases whose name
+    starts with 'full'), one line per output tensor."""
+    pu = sys.modules.get("parity_util")
+    recs = [r for r in getattr(pu, "ACCURACY", []) if r["case"].startswith("full")]
+    if not recs:
+        return
+    tr = terminalreporter
+    tr.write_sep("-", "accuracy against the oracle at full size (relative errors; face: mismatching ids)")
+    for r in recs:
+        if r["output"] == "face":
+            tr.write_line(f"{r['case']:42s} face    mismatches {int(r['maxnorm'])} of {r['n']}")
+        else:
+            tr.write_line(f"{r['case']:42s} {r['output']:7s} maxnorm {r['maxnorm']:.1e} elem max {r['elem_max']:.1e} "
+                          f"p99.9 {r['elem_p999']:.1e} median {r['elem_median']:.1e}")
