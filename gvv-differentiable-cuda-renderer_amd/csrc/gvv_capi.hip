@@ -87,14 +87,6 @@ void build_uv_table(const Handle& h, int texW, int texH, std::vector<UvTexel>& t
   }
 }
 
-int check_device(const Handle* h) {
-  int dev = -1;
-  GVV_CHECK(hipGetDevice(&dev));
-  if (dev != h->device) return fail("handle was created on device " + std::to_string(h->device) +
-                                    " but the current device is " + std::to_string(dev));
-  return 0;
-}
-
 inline int tiles(int n, int tile) { return (n + tile - 1) / tile; }
 
 }  // namespace
@@ -149,33 +141,60 @@ int plan_forward(Handle* h, int B, int texH, int texW, const float* vpos, const 
   return 0;
 }
 
-int plan_backward(Handle* h, int B, int texH, int texW, const float* g_render, const float* g_target,
-                  const float* target, const float* vpos, const float* vcol, const float* tex, const float* sh,
-                  const float* vnormal, const float* bary, const int32_t* face, const float* extr,
-                  const float* intr, float* g_pos, float* g_col, float* g_tex, float* g_sh, BackwardPlan& p) {
+int plan_backward_into(Handle* h, int B, int texH, int texW, const float* g_render, const float* g_target,
+                       const float* target, const float* vpos, const float* vcol, const float* tex, const float* sh,
+                       const float* vnormal, const float* bary, const int32_t* face, const float* extr,
+                       const float* intr, float* g_pos, float* g_col, float* g_tex, float* g_sh, Camera* cams,
+                       float* g_vn, float* sh_partial, BackwardPlan& p) {
   if (B <= 0) return fail("batch size must be > 0");
   if (texH <= 0 || texW <= 0) return fail("texture height/width must be > 0");
   if (!g_pos || !g_col || !g_tex || !g_sh) return fail("null gradient output pointer");
   if (h->albedo == kNormal || h->albedo == kLighting) return fail("normal and lighting modes have no gradients");
-  const int64_t nv = (int64_t)B * h->N * 3;
   if (!g_render || !vpos || !vcol || !tex || !sh || !vnormal || !face || !extr || !intr)
     return fail("null tensor pointer");
   if (g_target && !target) return fail("target_buffer_grad given without target_image");
   const int tx = tiles(h->W, h->tile), ty = tiles(h->H, h->tile);
-  GVV_CHECK(h->cams.reserve((size_t)B * h->C));
-  GVV_CHECK(h->g_vn.reserve((size_t)nv));
-  GVV_CHECK(h->sh_partial.reserve((size_t)B * h->C * tx * ty * 27));
   p.albedo = h->albedo;
   p.shaded = h->shading == kShaded;
-  p.extr = extr; p.intr = intr; p.cams = h->cams.p;
+  p.extr = extr; p.intr = intr; p.cams = cams;
   p.prof = nullptr;
   BackwardArgs& a = p.a;
   a.B = B; a.C = h->C; a.N = h->N; a.F = h->F; a.W = h->W; a.H = h->H; a.texW = texW; a.texH = texH;
   a.image_fs = h->image_fs; a.tile = h->tile; a.tiles_x = tx; a.tiles_y = ty;
   a.g_render = g_render; a.g_target = g_target; a.target = target;
   a.vpos = vpos; a.vcol = vcol; a.tex = tex; a.sh = sh; a.vnormal = vnormal; a.bary = bary; a.face = face;
-  a.faces = h->d_faces.p; a.uv = h->d_uv.p; a.cams = h->cams.p;
-  a.g_pos = g_pos; a.g_col = g_col; a.g_tex = g_tex; a.g_vn = h->g_vn.p; a.sh_partial = h->sh_partial.p; a.g_sh = g_sh;
+  a.faces = h->d_faces.p; a.uv = h->d_uv.p; a.cams = cams;
+  a.g_pos = g_pos; a.g_col = g_col; a.g_tex = g_tex; a.g_vn = g_vn; a.sh_partial = sh_partial; a.g_sh = g_sh;
+  return 0;
+}
+
+BackwardScratch backward_scratch(const Handle* h, int B) {
+  BackwardScratch s;
+  s.cams = (size_t)B * h->C;
+  s.g_vn = (size_t)B * h->N * 3;
+  s.sh_partial = (size_t)B * h->C * tiles(h->W, h->tile) * tiles(h->H, h->tile) * 27;
+  return s;
+}
+
+int plan_backward(Handle* h, int B, int texH, int texW, const float* g_render, const float* g_target,
+                  const float* target, const float* vpos, const float* vcol, const float* tex, const float* sh,
+                  const float* vnormal, const float* bary, const int32_t* face, const float* extr,
+                  const float* intr, float* g_pos, float* g_col, float* g_tex, float* g_sh, BackwardPlan& p) {
+  if (B > 0) {
+    const BackwardScratch n = backward_scratch(h, B);
+    GVV_CHECK(h->cams.reserve(n.cams));
+    GVV_CHECK(h->g_vn.reserve(n.g_vn));
+    GVV_CHECK(h->sh_partial.reserve(n.sh_partial));
+  }
+  return plan_backward_into(h, B, texH, texW, g_render, g_target, target, vpos, vcol, tex, sh, vnormal, bary, face,
+                            extr, intr, g_pos, g_col, g_tex, g_sh, h->cams.p, h->g_vn.p, h->sh_partial.p, p);
+}
+
+int check_device(const Handle* h) {
+  int dev = -1;
+  GVV_CHECK(hipGetDevice(&dev));
+  if (dev != h->device) return fail("handle was created on device " + std::to_string(h->device) +
+                                    " but the current device is " + std::to_string(dev));
   return 0;
 }
 

@@ -72,4 +72,20 @@ int plan_backward(Handle* h, int B, int texH, int texW, const float* g_render, c
                   const float* vnormal, const float* bary, const int32_t* face, const float* extr,
                   const float* intr, float* g_pos, float* g_col, float* g_tex, float* g_sh, BackwardPlan& p);
 
+// Element counts of the backward's scratch buffers for batch B: cameras, dL/d(vertex normal)
+// and the SH tile partials.
+struct BackwardScratch { size_t cams, g_vn, sh_partial; };
+BackwardScratch backward_scratch(const Handle* h, int B);
+
+// plan_backward with caller-provided scratch (sized by backward_scratch): validates and fills
+// the plan, allocates nothing.
+int plan_backward_into(Handle* h, int B, int texH, int texW, const float* g_render, const float* g_target,
+                       const float* target, const float* vpos, const float* vcol, const float* tex, const float* sh,
+                       const float* vnormal, const float* bary, const int32_t* face, const float* extr,
+                       const float* intr, float* g_pos, float* g_col, float* g_tex, float* g_sh, Camera* cams,
+                       float* g_vn, float* sh_partial, BackwardPlan& p);
+
+// 0 when the current HIP device is the one the handle was created on; else 1 with the message set.
+int check_device(const Handle* h);
+
 }  // namespace gvv
