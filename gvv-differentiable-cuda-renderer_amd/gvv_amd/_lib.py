@@ -13,6 +13,7 @@ _PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.environ.get("GVV_AMD_LIB", os.path.join(_PKG, "libgvv_amd.so"))
 
 EXPORTS = ("gvv_last_error", "gvv_version", "gvv_create", "gvv_destroy", "gvv_forward", "gvv_backward",
+           "gvv_backward_noalloc",
            "gvv_profile_enable", "gvv_profile_report", "gvv_set_tuning", "gvv_get_tuning",
            "gvv_calibrate_atomics",
            "gvv_conv_out_size", "gvv_conv_forward", "gvv_conv_backward_data", "gvv_conv_workspace_floats",
@@ -47,6 +48,8 @@ def load():
     L.gvv_forward.restype = ctypes.c_int
     L.gvv_backward.argtypes = [P, i32, i32, i32] + [P] * 17
     L.gvv_backward.restype = ctypes.c_int
+    L.gvv_backward_noalloc.argtypes = [P, i32, i32, i32] + [P] * 17
+    L.gvv_backward_noalloc.restype = ctypes.c_int
     L.gvv_profile_enable.argtypes = [P, i32]
     L.gvv_profile_enable.restype = ctypes.c_int
     L.gvv_profile_report.argtypes = [P, ctypes.c_char_p, i64]

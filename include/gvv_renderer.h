@@ -86,6 +86,22 @@ int gvv_backward(void* handle, int32_t B, int32_t texH, int32_t texW, const floa
                  const float* vnormal, const float* bary, const int32_t* face, const float* extr,
                  const float* intr, float* g_pos, float* g_col, float* g_tex, float* g_sh, void* hip_stream);
 
+/* gvv_backward without any device allocation.  No counterpart in the reference: for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, results and zero-gradient modes as
+ * gvv_backward.  The backward's scratch -- per-view cameras, the vertex-normal gradient and the
+ * SH tile partials, B*C*156 + B*N*12 + B*C*ceil(W/tile)*ceil(H/tile)*108 bytes, each part
+ * rounded up to 256 -- lives in one fixed 32 MiB device array per GPU that the library holds
+ * until the process ends; a call that would need more is rejected with a message (use
+ * gvv_backward or a smaller batch).  Every call
+ * on one GPU shares that GPU's array: calls that overlap on one GPU (other streams, other host
+ * threads, other handles) race and must be serialised by the caller. */
+int gvv_backward_noalloc(void* handle, int32_t B, int32_t texH, int32_t texW, const float* g_render,
+                         const float* g_target, const float* target, const float* vpos, const float* vcol,
+                         const float* tex, const float* sh, const float* vnormal, const float* bary,
+                         const int32_t* face, const float* extr, const float* intr, float* g_pos, float* g_col,
+                         float* g_tex, float* g_sh, void* hip_stream);
+
 /* Launch settings of one handle (no counterpart in the reference; chosen by the autotuner of
  * the multi-mesh runtime, gvv_multi.h).  tile: pixel tile edge of the backward pixel kernel,
  * 8, 16 or 32 (default 16).  large_box: bounding-box size in pixels above which the raster
