@@ -15,6 +15,7 @@
 //                  dL/d(vertex normal) per vertex; one thread per face then forms
 //                  G = sum of its vertices' normal gradients and scatters G x (vj - vi),
 //                  (vk - vi) x G and -(both) -- the same sum, regrouped by face.
+#include "gvv_bary_grad.hpp"
 #include "gvv_kernels.hpp"
 
 namespace gvv {
@@ -56,98 +57,17 @@ __device__ __forceinline__ void merge_pairs(float* t, int& slot, int lane) {
   if (merge && !low) slot = -1;
 }
 
-// Double-precision 3-vector for the barycentric Jacobian below.
-struct D3 { double x, y, z; };
-__device__ __forceinline__ D3 d3(double x, double y, double z) { D3 r; r.x = x; r.y = y; r.z = z; return r; }
-__device__ __forceinline__ D3 d3(V3 a) { return d3(a.x, a.y, a.z); }
-__device__ __forceinline__ D3 operator+(D3 a, D3 b) { return d3(a.x + b.x, a.y + b.y, a.z + b.z); }
-__device__ __forceinline__ D3 operator-(D3 a, D3 b) { return d3(a.x - b.x, a.y - b.y, a.z - b.z); }
-__device__ __forceinline__ D3 operator*(double s, D3 a) { return d3(s * a.x, s * a.y, s * a.z); }
-__device__ __forceinline__ double dot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ __forceinline__ D3 cross(D3 a, D3 b) { return d3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
-
-// d(a,b,c)/d(v0,v1,v2) for a fixed ray (reference dJBCDVerpos, RendererUtil.h:670-861),
-// contracted with the row vector gb (gradient w.r.t. the three barycentrics); adds the
-// 9-vector gradient w.r.t. (v0, v1, v2) to out[OFF .. OFF+8].
-//
-// Regrouped form of the reference's per-variable loop.  With e1 = v1-v0, e2 = v2-v0,
-// N = e1 x e2, q0 = P - v0 (P the plane hit of the ray), q1 = q0 - e1, q2 = q0 - e2, the edges
-// E0 = v2-v1 and E1 = v0-v2, C_r = E_r x q_{r+1}, and w_r = gb_r - gb_2 (the third row is minus
-// the sum of the first two), the contribution of vertex v with A_0 = e1-e2, A_1 = e2, A_2 = -e1
-// (dN/d(v . e_k) = e_k x A_v) is
-//   [A_v x (w0 C0 + w1 C1) + Q_v + dt_v (w0 N.(E0 x d) + w1 N.(E1 x d)) - D_v] / |N|^2
-//   - 2 (A_v x N) (w0 N.C0 + w1 N.C1) / |N|^4
-// with dt_v = (delta_v0 N - A_v x q0) / (N.d) the derivative of the ray parameter,
-// Q_0 = w1 q2 x N, Q_1 = -w0 q1 x N, Q_2 = (w0 q1 - w1 q2) x N from the edge derivatives,
-// and D_1 = w0 N x E0, D_2 = w1 N x E1 (D_0 = 0) from the vertex derivatives.
-//
-// Precision: the early exit is the reference's binary32 test.  The plane hit q0 and N.d are
-// ill-conditioned near grazing incidence (relative error ~eps/cos in binary32, which showed
-// as 2.8e-4 max-norm position-gradient error on a 5k-face sphere), so those two are formed
-// in FP64 from exact vertex differences; everything after them is well conditioned and stays
-// binary32 (1.3e-7 max-norm against the FP64 oracle on the same sphere).
-// Arithmetic of continuous gradient terms.  Binary32 rounding is part of the contract only for
-// the visibility and decision values (barycentrics, texel indices, the normal flip, early
-// exits); terms that only flow into gradients are checked against the FP64 oracle with a
-// relative tolerance, so in this type's operations products and sums may fuse into FMAs.
-#pragma clang fp contract(fast)
-struct F3 { float x, y, z; };
-GVV_HD F3 f3(V3 v) { F3 r; r.x = v.x; r.y = v.y; r.z = v.z; return r; }
-GVV_HD F3 f3(float x, float y, float z) { F3 r; r.x = x; r.y = y; r.z = z; return r; }
-GVV_HD F3 operator+(F3 a, F3 b) { return f3(a.x + b.x, a.y + b.y, a.z + b.z); }
-GVV_HD F3 operator-(F3 a, F3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
-GVV_HD F3 operator-(F3 a) { return f3(-a.x, -a.y, -a.z); }
-GVV_HD F3 operator*(float s, F3 a) { return f3(s * a.x, s * a.y, s * a.z); }
-GVV_HD F3 operator*(F3 a, float s) { return f3(s * a.x, s * a.y, s * a.z); }
-GVV_HD float dot(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-GVV_HD F3 cross(F3 a, F3 b) { return f3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
-
+// d(a,b,c)/d(v0,v1,v2) for a fixed ray, contracted with the row vector gb (gradient w.r.t. the
+// three barycentrics), added to out[OFF .. OFF+8] (gvv_bary_grad.hpp; reference dJBCDVerpos,
+// RendererUtil.h:670-861).  Binary32 throughout: the ill-conditioned plane hit is formed with
+// error-free transformations.
 template <int OFF>
-__device__ __forceinline__ void bary_position_grad(V3 of, V3 d, V3 v0f, V3 v1f, V3 v2f, V3 gb, float* out) {
-  {
-    // the reference's binary32 early exit (a decision)
-    const V3 e1 = v1f - v0f, e2 = v2f - v0f;
-    const V3 N = cross(e1, e2);
-    const float denf = dot(N, N);
-    if (fabsf(dot(normalize(d), normalize(N))) < 0.001f || fabsf(denf * denf) < 0.001f) return;
-  }
-  const F3 e1 = f3(v1f - v0f), e2 = f3(v2f - v0f), df = f3(d);
-  const F3 N = cross(e1, e2);
-  F3 q0;
-  float nd;
-  {
-    const D3 v0 = d3(v0f), dd = d3(d);
-    const D3 Nd = cross(d3(v1f) - v0, d3(v2f) - v0);
-    const D3 w = v0 - d3(of);
-    const double ndd = dot(Nd, dd);
-    const D3 q = (dot(w, Nd) / ndd) * dd - w;
-    q0 = f3((float)q.x, (float)q.y, (float)q.z);
-    nd = (float)ndd;
-  }
-  const float den = dot(N, N);
-  const F3 q1 = q0 - e1, q2 = q0 - e2;
-  const F3 E0 = e2 - e1, E1 = -e2;
-  const float w0 = gb.x - gb.z, w1 = gb.y - gb.z;
-  const F3 C0 = cross(E0, q1), C1 = cross(E1, q2);
-  const F3 Cw = w0 * C0 + w1 * C1;
-  const float NCw = w0 * dot(N, C0) + w1 * dot(N, C1);
-  const float NEdw = w0 * dot(N, cross(E0, df)) + w1 * dot(N, cross(E1, df));
-  const float inv_den = __builtin_amdgcn_rcpf(den), inv_nd = __builtin_amdgcn_rcpf(nd);
-  const float c2 = -2.f * NCw * inv_den * inv_den;
-#pragma unroll
-  for (int v = 0; v < 3; ++v) {
-    const F3 A = v == 0 ? e1 - e2 : (v == 1 ? e2 : -e1);
-    F3 dt = cross(A, q0) * (-inv_nd);
-    if (v == 0) dt = dt + N * inv_nd;
-    F3 Q = v == 0 ? cross(w1 * q2, N) : (v == 1 ? cross(-w0 * q1, N) : cross(w0 * q1 - w1 * q2, N));
-    if (v == 1) Q = Q - w0 * cross(N, E0);
-    if (v == 2) Q = Q - w1 * cross(N, E1);
-    const F3 g = inv_den * (cross(A, Cw) + Q + NEdw * dt) + c2 * cross(A, N);
-    out[OFF + 3 * v + 0] += g.x;
-    out[OFF + 3 * v + 1] += g.y;
-    out[OFF + 3 * v + 2] += g.z;
-  }
+__device__ __forceinline__ void bary_position_grad(V3 o, V3 d, V3 v0, V3 v1, V3 v2, V3 gb, float* out) {
+  const BaryGradSetup g = bary_grad_setup(o, d, v0, v1, v2);
+  if (!g.live) return;
+  bary_grad_apply(g, d, gb, __builtin_amdgcn_rcpf(g.den), __builtin_amdgcn_rcpf(g.nd), out + OFF);
 }
+#pragma clang fp contract(fast)
 // SH basis of a unit normal (the reference's order of the nine terms) and the illumination
 // it gives under the coefficients of one view, for the continuous terms of the backward.
 GVV_HD void sh_basis(V3 n, float Y[9]) {
