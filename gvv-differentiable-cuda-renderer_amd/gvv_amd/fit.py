@@ -181,6 +181,15 @@ def plain_init(B: int, n_basis: int, device) -> Params:
     return Params(torch.zeros((B, n_basis), device=device), sh, torch.zeros((B, 6), device=device))
 
 
+def plain_subject_losses(model: "ShapeModel", target, mask):
+    """Per-subject masked loss [B] of the plain start.  The plain start is the same for every
+    subject (zero shape and pose, neutral lighting), so one subject's render serves all."""
+    with torch.no_grad():
+        r = model.render(plain_init(1, model.cfg.n_basis, model.device))[0]
+        d = (r - target) * mask
+        return (d * d).flatten(1).sum(1) / mask.flatten(1).sum(1).clamp_min(1.0)
+
+
 def masked_l2(render, target, mask):
     """Mean squared photometric error over the target's foreground pixels, per subject summed."""
     d = (render - target) * mask
@@ -191,7 +200,8 @@ def masked_l2(render, target, mask):
 class FitResult:
     params: Params
     losses: list = field(default_factory=list)
-    times: list = field(default_factory=list)          # wall-clock seconds after each iteration
+    times: list = field(default_factory=list)          # wall-clock seconds when each loss was known
+    fallbacks: int = 0
 
     @property
     def final_loss(self):
@@ -205,10 +215,17 @@ class FitResult:
         return None
 
 
-def fit(model: ShapeModel, target, mask, init: Params, iters=None, lr=None, stop_at=None, t0=None) -> FitResult:
+def fit(model: ShapeModel, target, mask, init: Params, iters=None, lr=None, stop_at=None, t0=None,
+        fallback=None) -> FitResult:
     """Adam over (shape, SH, pose).  `losses[i]` is the loss of the parameters before step i+1;
-    `times[i]` the wall-clock since t0 (default: entry) after evaluating it.  Stops early once
-    the loss is <= stop_at."""
+    `times[i]` the wall-clock since t0 (default: entry) once that loss is known.  Stops as soon
+    as the loss is <= stop_at (before that iteration's backward).
+
+    `fallback` = (plain parameters, their per-subject losses [B]): the first iteration's forward
+    also yields each subject's own loss at `init`; a subject whose loss there is worse than its
+    plain loss falls back to the plain parameters (its first update is skipped, as its gradient
+    was taken at the discarded start) and the recorded first loss is that of the chosen starts.
+    `FitResult.fallbacks` counts them."""
     cfg = model.cfg
     iters = cfg.iters if iters is None else iters
     lr = cfg.lr if lr is None else lr
@@ -216,16 +233,35 @@ def fit(model: ShapeModel, target, mask, init: Params, iters=None, lr=None, stop
     opt = torch.optim.Adam([x], lr=lr)
     res = FitResult(params=None)
     t0 = time.perf_counter() if t0 is None else t0
-    for _ in range(iters):
+    mask_sub = mask.flatten(1).sum(1)
+    for it in range(iters):
         opt.zero_grad(set_to_none=True)
         render, _ = model.render(Params.from_flat(x, cfg.n_basis))
-        loss = masked_l2(render, target, mask)
-        loss.backward()
-        lv = float(loss.detach())
+        d = (render - target) * mask
+        sq = (d * d).flatten(1).sum(1)                       # per subject
+        loss = sq.sum() / mask_sub.sum().clamp_min(1.0)
+        back = None
+        if it == 0 and fallback is not None:
+            plain, plain_losses = fallback
+            own = sq.detach() / mask_sub.clamp_min(1.0)
+            back = own > plain_losses
+            res.fallbacks = int(back.sum())
+            if res.fallbacks:
+                chosen = torch.where(back, plain_losses * mask_sub, sq.detach())
+                lv = float(chosen.sum() / mask_sub.sum().clamp_min(1.0))
+            else:
+                lv = float(loss.detach())
+        else:
+            lv = float(loss.detach())
         res.losses.append(lv)
         res.times.append(time.perf_counter() - t0)
         if stop_at is not None and lv <= stop_at:
             break
+        loss.backward()
+        if back is not None and res.fallbacks:
+            with torch.no_grad():
+                x.grad[back] = 0.0
+                x[back] = plain.flat()[back]
         opt.step()
     res.params = Params.from_flat(x.detach(), cfg.n_basis)
     return res

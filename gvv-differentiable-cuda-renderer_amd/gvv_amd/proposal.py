@@ -318,19 +318,42 @@ class WarmStartReport:
     warm_seconds: float | None          # until the warm fit reached plain_final_loss (incl. proposal)
     warm_iters: int | None
     fallbacks: int
+    warm_runs: list | None = None       # (seconds, iterations) of every warm run; warm_seconds is the median
+    plain_runs: list | None = None      # (seconds, final loss) of every plain run
 
     @property
     def ratio(self):
         return None if self.warm_seconds is None else self.warm_seconds / self.plain_seconds
 
+    @property
+    def ratios(self):
+        return [None if w is None else w / self.plain_seconds for w, _ in (self.warm_runs or [])]
+
+
+def warm_fit(net: ProposalNet, model: fit.ShapeModel, target, mask, iters: int, level: float):
+    """The warm-started fit, timed from the proposal on: the network's prediction from the
+    target images, the plain start's per-subject losses (one single-subject render), then the
+    fit, which takes each subject's loss at the proposal from its own first forward and falls
+    back to the plain start where that is worse."""
+    t0 = time.perf_counter()
+    prop = propose(net, model, target)
+    B = target.shape[0]
+    plain = fit.plain_init(B, model.cfg.n_basis, model.device)
+    res = fit.fit(model, target, mask, prop, iters=iters, stop_at=level, t0=t0,
+                  fallback=(plain, fit.plain_subject_losses(model, target, mask)))
+    torch.cuda.synchronize()
+    it = next((i for i, l in enumerate(res.losses) if l <= level), None)
+    return res, res.time_to(level), it
+
 
 def compare_fits(net: ProposalNet, model: fit.ShapeModel, gt: fit.Params, iters: int | None = None,
-                 plain_runs: int = 3) -> WarmStartReport:
+                 plain_runs: int = 3, warm_runs: int = 3) -> WarmStartReport:
     """Plain fit for `iters` Adam steps (`plain_runs` times: float atomics make its last bits, and
     so its final loss, vary from run to run; the run with the median final loss is the
-    reference), then the warm-started fit until it reaches that run's final loss (at most the
-    same number of steps).  Both clocks include everything done on the way (the warm one
-    includes the proposal, its fallback check and packing)."""
+    reference), then the warm-started fit (`warm_runs` times) until it reaches that run's final
+    loss (at most the same number of steps).  Both clocks include everything done on the way
+    (the warm one includes the proposal, its fallback check and packing); the reported warm time
+    is the median run's."""
     target, mask = fit.make_targets(model, gt)
     B = target.shape[0]
     iters = model.cfg.iters if iters is None else iters
@@ -340,20 +363,17 @@ def compare_fits(net: ProposalNet, model: fit.ShapeModel, gt: fit.Params, iters:
     for _ in range(max(1, plain_runs)):
         runs.append(fit.fit(model, target, mask, fit.plain_init(B, model.cfg.n_basis, model.device), iters=iters))
         torch.cuda.synchronize()
-    runs.sort(key=lambda r: r.final_loss)
-    plain = runs[len(runs) // 2]
+    order = sorted(range(len(runs)), key=lambda i: runs[i].final_loss)
+    plain = runs[order[len(runs) // 2]]
     level = plain.final_loss
-    t0 = time.perf_counter()
-    init, fallbacks = warm_start(net, model, target, mask)
-    warm = fit.fit(model, target, mask, init, iters=iters, stop_at=level, t0=t0)
-    torch.cuda.synchronize()
-    t_warm = warm.time_to(level)
-    it_warm = None
-    for i, l in enumerate(warm.losses):
-        if l <= level:
-            it_warm = i
-            break
+    warm = []
+    for _ in range(max(1, warm_runs)):
+        warm.append(warm_fit(net, model, target, mask, iters, level))
+    timed = sorted(warm, key=lambda w: float("inf") if w[1] is None else w[1])
+    res, t_warm, it_warm = timed[len(timed) // 2]
     return WarmStartReport(plain_final_loss=level, plain_seconds=plain.times[-1],
                            plain_seconds_to_final=plain.time_to(level), plain_iters=iters,
-                           warm_initial_loss=warm.losses[0], plain_initial_loss=plain.losses[0],
-                           warm_seconds=t_warm, warm_iters=it_warm, fallbacks=fallbacks)
+                           warm_initial_loss=res.losses[0], plain_initial_loss=plain.losses[0],
+                           warm_seconds=t_warm, warm_iters=it_warm, fallbacks=res.fallbacks,
+                           warm_runs=[(w[1], w[2]) for w in warm],
+                           plain_runs=[(r.times[-1], r.final_loss) for r in runs])

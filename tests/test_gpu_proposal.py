@@ -70,16 +70,39 @@ def small_model(device):
     return fit.ShapeModel(cfg, device)
 
 
-def test_training_beats_the_prior_mean(device):
-    """The loss is the MSE of normalised coefficients, so predicting the prior mean scores 1.
-    A short training run on rasterizer renders must do clearly better than that."""
+@pytest.fixture(scope="module")
+def trained(device):
+    """A short training run on rasterizer renders of the reduced fit workload."""
     model = small_model(device)
     net = proposal.ProposalNet(device, seed=0)
     log = proposal.train(net, model, samples=32 * 300, batch=32, lr=1e-3, seed=5, log_every=1)
+    return model, net, log
+
+
+def test_training_beats_the_prior_mean(trained):
+    """The loss is the MSE of normalised coefficients, so predicting the prior mean scores 1.
+    A short training run on rasterizer renders must do clearly better than that."""
+    model, net, log = trained
     first = np.mean([l for _, l in log.losses[:5]])
     last = np.mean([l for _, l in log.losses[-20:]])
     print(f"training loss {first:.4f} -> {last:.4f} in {log.seconds:.1f}s", flush=True)
     assert math.isfinite(last) and last < 0.9
+
+
+def test_trained_network_gains_over_the_plain_fit(device, trained):
+    """What the trained network gains over the method without it, on held-out subjects (a
+    seed the training never drew): the warm start begins below the plain start's loss, and the
+    warm-started fit reaches the plain fit's final loss (after 100 Adam steps) in far fewer
+    steps than the plain fit takes."""
+    model, net, _ = trained
+    gt = fit.sample_params(np.random.Generator(np.random.PCG64(424242)), model.cfg.batch, model.cfg.n_basis, device)
+    rep = proposal.compare_fits(net, model, gt, iters=100, plain_runs=1, warm_runs=1)
+    print(f"plain: loss {rep.plain_initial_loss:.4f} -> {rep.plain_final_loss:.4f} in {rep.plain_iters} steps, "
+          f"{rep.plain_seconds:.3f}s; warm: starts at {rep.warm_initial_loss:.4f}, reaches it after "
+          f"{rep.warm_iters} steps, {rep.warm_seconds}s (ratio {rep.ratio}), fallbacks {rep.fallbacks}", flush=True)
+    assert rep.warm_initial_loss < rep.plain_initial_loss
+    assert rep.warm_iters is not None and rep.warm_iters <= 0.6 * rep.plain_iters
+    assert rep.ratio is not None and rep.ratio < 0.8
 
 
 def test_warm_start_fallback_never_worse(device):
@@ -95,3 +118,9 @@ def test_warm_start_fallback_never_worse(device):
     lp = proposal.subject_losses(model, plain, target, mask)
     print("fallbacks", fallbacks, li.tolist(), lp.tolist(), flush=True)
     assert torch.all(li <= lp)
+    # the fit's own fallback (decided from its first forward) starts no worse than plain either
+    res, _, _ = proposal.warm_fit(net, model, target, mask, iters=3, level=-1.0)
+    plain_loss = float(fit.masked_l2(model.render(plain)[0].detach(), target, mask))
+    print("fit fallbacks", res.fallbacks, "first loss", res.losses[0], "plain", plain_loss, flush=True)
+    assert res.fallbacks == fallbacks
+    assert res.losses[0] <= plain_loss * (1 + 1e-5)
