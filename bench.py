@@ -106,9 +106,12 @@ def cpu_model():
     return "unknown"
 
 
-def cpu_baseline(scene, threads, views=8, reps=3):
+def cpu_baseline(scene, threads, views=8, reps=3, g_render=None):
     """Oracle forward + backward over `views` of the views, multithreaded: one warm-up, then the
-    median of `reps` timed repetitions (one-time setup excluded)."""
+    median of `reps` timed repetitions (one-time setup excluded).  With all views and a given
+    upstream gradient `g_render`, the backward takes that gradient (the one the GPU step used),
+    so that the last repetition's outputs can be compared with the GPU's.  Returns the baseline
+    dict and (forward outputs, gradients) of the last repetition."""
     import numpy as np
     from oracle import oracle as O
     O.set_threads(threads)
@@ -120,14 +123,18 @@ def cpu_baseline(scene, threads, views=8, reps=3):
            np.ascontiguousarray(scene.extr[:, :views]), np.ascontiguousarray(scene.intr[:, :views]))
     target = np.ascontiguousarray(scene.target[:, :views])
     times = []
+    out = gout = None
     for rep in range(reps + 1):
         t0 = time.perf_counter()
         out = orc.forward(*ins)
         t1 = time.perf_counter()
-        g = (2.0 * (out["render"] - target)).astype(np.float32)
+        if g_render is not None and views == scene.C:
+            g = g_render
+        else:
+            g = (2.0 * (out["render"] - target)).astype(np.float32)
         t2 = time.perf_counter()
-        orc.backward(g, np.zeros_like(g), target, scene.vpos, scene.vcol, scene.tex, ins[3], out["vnormal"], out["bary"],
-                     out["face"], ins[4], ins[5])
+        gout = orc.backward(g, np.zeros_like(g), target, scene.vpos, scene.vcol, scene.tex, ins[3], out["vnormal"],
+                            out["bary"], out["face"], ins[4], ins[5])
         t3 = time.perf_counter()
         if rep > 0:
             times.append((t1 - t0, t3 - t2))
@@ -136,7 +143,67 @@ def cpu_baseline(scene, threads, views=8, reps=3):
     px = views * scene.width * scene.height * scene.B
     return dict(value=px / (t_fwd + t_bwd) / 1e6, unit="Mpix/s", cores=threads, kind="port",
                 sample=f"FP64 oracle fwd+bwd on {views} of {scene.C} views ({px} px, {scene.F} triangles), median of "
-                       f"{reps} after 1 warm-up: fwd {t_fwd:.2f}s bwd {t_bwd:.2f}s; {threads} threads on {cpu_model()}")
+                       f"{reps} after 1 warm-up: fwd {t_fwd:.2f}s bwd {t_bwd:.2f}s; {threads} threads on {cpu_model()}"), \
+        (out, gout, views == scene.C)
+
+
+# accuracy bar of the path: ids, barycentrics and vertex normals bit-exact; render within 1e-5
+# and gradients within 1e-4 max-norm relative of the FP64 oracle
+RENDER_TOL, GRAD_TOL, ELEM_FLOOR = 1e-5, 1e-4, 1e-3
+
+
+def rel_errors(x, ref):
+    """Max-norm relative error and, over the entries with |ref| >= 1e-3 max|ref|, the max,
+    99.9th percentile and median of the elementwise relative error."""
+    import numpy as np
+    x = np.asarray(x, np.float64).ravel()
+    ref = np.asarray(ref, np.float64).ravel()
+    amax = float(np.abs(ref).max()) if ref.size else 0.0
+    diff = float(np.abs(x - ref).max()) if ref.size else 0.0
+    r = dict(maxnorm=diff / amax if amax > 0 else diff)
+    if amax > 0:
+        m = np.abs(ref) >= ELEM_FLOOR * amax
+        e = np.abs(x[m] - ref[m]) / np.abs(ref[m])
+        r.update(elem_max=float(e.max()), elem_p999=float(np.quantile(e, 0.999)), elem_median=float(np.median(e)))
+    return {k: float(f"{v:.3g}") for k, v in r.items()}
+
+
+def accuracy_report(gpu, ref, gref):
+    """Per-output errors of the GPU step's outputs against the oracle's on the same inputs."""
+    import numpy as np
+    acc = dict(face_mismatch=int((gpu["face"] != ref["face"]).sum()),
+               bary_bit_exact=bool(np.array_equal(gpu["bary"].view(np.uint32), ref["bary"].view(np.uint32))),
+               vnormal_bit_exact=bool(np.array_equal(gpu["vnormal"].view(np.uint32), ref["vnormal"].view(np.uint32))),
+               render=rel_errors(gpu["render"], ref["render"]))
+    for k in ("pos", "col", "tex", "sh"):
+        acc["g_" + k] = rel_errors(gpu["g_" + k], gref[k])
+    acc["tolerances"] = dict(render=RENDER_TOL, grad=GRAD_TOL, elementwise_floor=ELEM_FLOOR)
+    acc["meets_tolerances"] = bool(acc["face_mismatch"] == 0 and acc["bary_bit_exact"] and acc["vnormal_bit_exact"]
+                                   and acc["render"]["maxnorm"] <= RENDER_TOL
+                                   and all(acc["g_" + k]["maxnorm"] <= GRAD_TOL for k in ("pos", "col", "tex", "sh")))
+    return acc
+
+
+def load_counters(workload):
+    """Counter figures of the kernels recorded with rocprofv3 for this workload (copy-calibrated
+    HBM traffic, VALU instructions and atomic requests per launch), with the launch settings and
+    kernel-source digest they were recorded at; None if absent."""
+    f = os.path.join(ROOT, "profiles", f"counters_{workload}.json")
+    if not os.path.exists(f):
+        return None
+    with open(f) as fh:
+        return json.load(fh)
+
+
+def source_digest():
+    """Digest of the HIP sources of the library: counters recorded from other sources are stale."""
+    import hashlib
+    h = hashlib.sha256()
+    csrc = os.path.join(PKG, "csrc")
+    for name in sorted(os.listdir(csrc)):
+        with open(os.path.join(csrc, name), "rb") as fh:
+            h.update(name.encode() + b"\0" + fh.read())
+    return h.hexdigest()[:16]
 
 
 def step_bytes(scene, covered, texW, texH):
@@ -164,7 +231,12 @@ def fit_experiment(train_samples):
     log = proposal.train(net, model, samples=train_samples, batch=16, lr=7e-4, seed=1)
     gt = fit.sample_params(np.random.Generator(np.random.PCG64(12345)), cfg.batch, cfg.n_basis, dev)
     rep = proposal.compare_fits(net, model, gt)
+    ratios = [r for r in rep.ratios if r is not None]
     return dict(metric="fit time-to-tolerance vs plain", ratio=None if rep.ratio is None else round(rep.ratio, 4),
+                ratio_runs=[None if r is None else round(r, 4) for r in rep.ratios],
+                ratio_spread=round(max(ratios) - min(ratios), 4) if ratios else None,
+                warm_iters_runs=[it for _, it in rep.warm_runs],
+                plain_runs=[dict(seconds=round(t, 4), final_loss=l) for t, l in rep.plain_runs],
                 plain_seconds=round(rep.plain_seconds, 4), plain_seconds_to_final=round(rep.plain_seconds_to_final, 4),
                 plain_iters=rep.plain_iters,
                 plain_final_loss=rep.plain_final_loss, plain_initial_loss=rep.plain_initial_loss,
@@ -175,8 +247,10 @@ def fit_experiment(train_samples):
                             subjects=cfg.batch, views=cfg.views, resolution=cfg.resolution, adam_lr=cfg.lr))
 
 
-def bench_multi(args):
-    """Batched uneven workload through the multi-mesh runtime (graph replay, autotuned)."""
+def multi_figures(steps, warmup):
+    """Batched uneven workload (32 meshes, 2k-200k triangles, 1-8 views at 256/512/1024^2)
+    through the multi-mesh runtime: plain launches and one graph launch per step with default
+    settings, then the autotuner and the graph again.  Returns the figures as a dict."""
     import numpy as np
     import torch
     from gvv_amd import scenes
@@ -190,7 +264,7 @@ def bench_multi(args):
         b["g_render"].copy_(torch.from_numpy(rng.normal(size=tuple(b["g_render"].shape)).astype(np.float32)))
 
     def timed(steps):
-        for _ in range(args.warmup):
+        for _ in range(warmup):
             mr.run(backward=True)
         torch.cuda.synchronize()
         a, z = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -202,36 +276,44 @@ def bench_multi(args):
         return a.elapsed_time(z) / steps
 
     mr.use_graph(False)
-    ms_plain_default = timed(args.steps)
+    ms_plain_default = timed(steps)
     mr.use_graph(True)
-    ms_graph_default = timed(args.steps)
+    ms_graph_default = timed(steps)
     t0 = time.perf_counter()
     mr.autotune(reps=3)
     tune_s = time.perf_counter() - t0
-    ms = timed(args.steps)
+    ms = timed(steps)
     P = mr.pixels()
     tuning = [mr.tuning(i) for i in range(len(batch))]
+    out = dict(value=round(P / (ms / 1e3) / 1e6, 2), unit="Mpix/s", ms_per_step=round(ms, 4), steps=steps,
+               workload="multi32_uneven", meshes=len(batch), pixels=P, triangle_views=mr.triangles(),
+               triangles_per_s=round(mr.triangles() / (ms / 1e3), 1),
+               ms_plain_launches_default_tuning=round(ms_plain_default, 4),
+               ms_graph_default_tuning=round(ms_graph_default, 4), autotune_seconds=round(tune_s, 2),
+               tuning=[dict(mesh=s.meta["name"], tile=t, large_box=lb, size_class=c)
+                       for s, (t, lb, c) in zip(batch, tuning)])
+    del mr
+    torch.cuda.synchronize()
+    return out
+
+
+def bench_multi(args):
+    """--workload multi: the batched uneven workload as the line's own metric."""
+    m = multi_figures(args.steps, args.warmup)
     line = {
         "metric": "fwd+bwd Mpix/s, batched uneven meshes (one graph launch per step)",
-        "value": round(P / (ms / 1e3) / 1e6, 2),
+        "value": m["value"],
         "unit": "Mpix/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms, 4),
+        "ms_per_step": m["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic (32 seeded icosphere/grid meshes)",
-        "config": {"workload": "multi32_uneven", "meshes": len(batch), "pixels": P,
-                   "triangle_views": mr.triangles(),
-                   "triangles_per_s": round(mr.triangles() / (ms / 1e3), 1),
-                   "ms_plain_launches_default_tuning": round(ms_plain_default, 4),
-                   "ms_graph_default_tuning": round(ms_graph_default, 4),
-                   "autotune_seconds": round(tune_s, 2),
-                   "tuning": [dict(mesh=s.meta["name"], tile=t, large_box=lb, size_class=c)
-                              for s, (t, lb, c) in zip(batch, tuning)]},
+        "config": {k: v for k, v in m.items() if k not in ("value", "unit", "ms_per_step", "steps")},
     }
     print(json.dumps(line), flush=True)
 
@@ -249,6 +331,7 @@ def main():
     ap.add_argument("--kernels-json", default=None, help="also write per-kernel timings here")
     ap.add_argument("--workload", choices=("render", "multi"), default="render")
     ap.add_argument("--no-fit", action="store_true", help="skip the proposal + fit time-to-tolerance run")
+    ap.add_argument("--no-multi", action="store_true", help="skip the batched uneven-mesh figures")
     ap.add_argument("--train-samples", type=int, default=200_000)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
                     help="after timing, run one plain device-to-device copy of MB MiB (reference kernel for "
@@ -332,30 +415,48 @@ def main():
     dom = max(kernels, key=lambda k: kernels[k][0])
     dom_ms = kernels[dom][0] / kernels[dom][1]
     dom_bytes = kernel_bytes(dom, scene, covered, texW, texH)
+    workload = "cylinder100k_8x1024"
+    counters = load_counters(workload)
+    stamp = None
+    if counters is not None:
+        rec = counters.get("recorded", {})
+        digest = source_digest()
+        stale = []
+        if list(rec.get("tuning", [])) != [best[0], best[1]]:
+            stale.append(f"recorded at tuning {rec.get('tuning')}, this run chose {list(best)}")
+        if rec.get("sources") != digest:
+            stale.append(f"recorded from kernel sources {rec.get('sources')}, these are {digest}")
+        for msg in stale:
+            print(f"bench.py: counter figures in profiles/counters_{workload}.json are stale: {msg}",
+                  file=sys.stderr, flush=True)
+        stamp = dict(file=f"profiles/counters_{workload}.json", tuning=rec.get("tuning"), commit=rec.get("commit"),
+                     sources=rec.get("sources"), stale=bool(stale))
     roofline = None
     if dom_bytes is not None:
         achieved = dom_bytes / (dom_ms / 1e3) / 1e9
-        traffic = None
-        tf = os.path.join(ROOT, "profiles", "traffic.json")
-        if os.path.exists(tf):
-            with open(tf) as fh:
-                traffic = json.load(fh).get(dom)
-        roofline = dict(bound="hbm", kernel=dom, achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                        frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, algorithmic_bytes=dom_bytes,
-                        avg_ms=round(dom_ms, 4))
-        vf = os.path.join(ROOT, "profiles", "valu.json")
-        if os.path.exists(vf):
-            with open(vf) as fh:
-                insts = json.load(fh).get(dom)
-            if insts:
-                # share of the wave64 VALU issue rate
-                roofline["valu_issue_frac"] = round(insts / (dom_ms / 1e3 * VALU_PEAK_PER_S), 4)
+        traffic = insts = None
+        if counters is not None:
+            traffic = counters.get("traffic_bytes", {}).get(dom)
+            insts = counters.get("valu_insts", {}).get(dom)
+        hbm = dict(achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4),
+                   traffic=traffic, algorithmic_bytes=dom_bytes,
+                   traffic_frac=None if traffic is None else round(traffic / (dom_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4))
+        valu = None
+        if insts:
+            rate = insts / (dom_ms / 1e3)
+            valu = dict(achieved=round(rate / 1e9, 1), peak=round(VALU_PEAK_PER_S / 1e9, 1), unit="Ginst/s",
+                        frac=round(rate / VALU_PEAK_PER_S, 4), insts=insts)
+        # the bound is the resource the kernel uses the larger share of: HBM (counter traffic, or
+        # the algorithmic bytes when no traffic was recorded) or VALU instruction issue
+        hbm_use = hbm["traffic_frac"] if hbm["traffic_frac"] is not None else hbm["frac"]
+        top = valu if valu is not None and valu["frac"] > hbm_use else hbm
+        roofline = dict(bound="valu_issue" if top is valu else "hbm", kernel=dom, achieved=top["achieved"],
+                        peak=top["peak"], unit=top["unit"], frac=top["frac"], traffic=traffic, avg_ms=round(dom_ms, 4),
+                        hbm=hbm, valu=valu, counters=stamp)
 
     atomics = None
-    af = os.path.join(ROOT, "profiles", "atomics.json")
-    if os.path.exists(af):
-        with open(af) as fh:
-            counts = json.load(fh)
+    if counters is not None and counters.get("atomic_requests"):
+        counts = counters["atomic_requests"]
         peak = atomic_peak(dev)
         atomics = {}
         for k in ("k_pixel_grad", "k_raster", "k_raster_large", "k_grad_finish"):
@@ -368,6 +469,7 @@ def main():
         # (3 texel + 27 SH + 9 barycentric-position + 9 model-to-data + ~162 per-incident-face normal terms)
         atomics["reference_formulation_per_step"] = 210 * covered
         atomics["issued_requests_per_step"] = sum(v for k, v in counts.items() if k in kernels)
+        atomics["counters"] = stamp
 
     if args.calibrate_copy > 0:
         n = args.calibrate_copy * (1 << 20) // 4
@@ -377,10 +479,31 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    cpu = None
+    # the outputs of one step at the chosen settings, for the accuracy check against the oracle
+    gb, gf, gr, gvn, _ = rast.forward(vpos, vcol, tex, sh, extr, intr)
+    gg = rast.backward(g_render, None, None, vpos, vcol, tex, sh, gvn, gf, extr, intr)
+    torch.cuda.synchronize()
+    gpu_out = dict(bary=gb.cpu().numpy(), face=gf.cpu().numpy(), render=gr.cpu().numpy(), vnormal=gvn.cpu().numpy(),
+                   **{"g_" + k: v.cpu().numpy() for k, v in zip(("pos", "col", "tex", "sh"), gg)})
+    g_render_np = g_render.cpu().numpy()
+    del gb, gf, gr, gvn, gg
+
+    cpu = accuracy = None
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        cpu = cpu_baseline(scene, threads, views=args.cpu_views, reps=args.cpu_reps)
+        cpu, (ref, gref, full) = cpu_baseline(scene, threads, views=args.cpu_views, reps=args.cpu_reps,
+                                              g_render=g_render_np)
+        if full:
+            accuracy = accuracy_report(gpu_out, ref, gref)
+        del ref, gref
+    del gpu_out
+
+    multi = None
+    if not args.no_multi:
+        try:
+            multi = multi_figures(args.steps, args.warmup)
+        except Exception as e:      # reported in the line; the throughput figure stands on its own
+            multi = dict(error=f"{type(e).__name__}: {e}")
 
     fit = None
     if not args.no_fit:
@@ -420,7 +543,9 @@ def main():
         "atomics": atomics,
         "cpu_baseline": cpu,
         "kernels_ms": {k: round(v[0] / v[1], 4) for k, v in kernels.items()},
+        "accuracy": accuracy,
         "fit": fit,
+        "multi": multi,
     }
     if args.kernels_json:
         with open(args.kernels_json, "w") as fh:
