@@ -112,6 +112,20 @@ int plan_forward(Handle* h, int B, int texH, int texW, const float* vpos, const 
   GVV_CHECK(h->large.reserve((size_t)B * h->C * h->F));
   if (h->keys.n != old_keys || old_keys == 0)
     GVV_CHECK(launch_fill_u64(h->keys.p, (int64_t)h->keys.n, kEmptyKey, s));
+  const bool tiled = h->depth_pass == 1;
+  const int tiles_x = tiles(h->W, kScreenTile), tiles_y = tiles(h->H, kScreenTile);
+  if (tiled) {
+    const size_t nt = (size_t)B * h->C * tiles_x * tiles_y;
+    const size_t old_t = h->tile_count.n;
+    GVV_CHECK(h->tile_count.reserve(nt));
+    GVV_CHECK(h->tile_dirty.reserve(nt));
+    GVV_CHECK(h->bins.reserve(nt * kBinCap));
+    if (h->tile_count.n != old_t || old_t == 0) {
+      // counts and marks start at zero; the tile kernel resets the ones it read
+      GVV_CHECK(launch_fill_f32(reinterpret_cast<float*>(h->tile_count.p), (int64_t)h->tile_count.n, 0.f, s));
+      GVV_CHECK(launch_fill_f32(reinterpret_cast<float*>(h->tile_dirty.p), (int64_t)h->tile_dirty.n, 0.f, s));
+    }
+  }
   if (h->normal_map && (h->uvW != texW || h->uvH != texH)) {
     std::vector<UvTexel> table;
     build_uv_table(*h, texW, texH, table);
@@ -130,6 +144,12 @@ int plan_forward(Handle* h, int B, int texH, int texW, const float* vpos, const 
   p.cams = h->cams.p; p.fnormal = h->fnormal.p; p.tris = h->tris.p; p.proj = h->proj.p;
   p.counters = h->counters.p; p.large = h->large.p; p.large_box = h->large_box;
   p.mesh_id = 0; p.shared_queue = false;
+  p.tiled = tiled;
+  p.tile_count = tiled ? h->tile_count.p : nullptr;
+  p.bins = tiled ? h->bins.p : nullptr;
+  p.marks.dirty = tiled ? h->tile_dirty.p : nullptr;
+  p.marks.tiles_x = tiles_x;
+  p.marks.tiles_y = tiles_y;
   p.vnormal_out = vnormal;
   p.normal_map_out = normal_map;
   ResolveArgs& r = p.r;
@@ -313,6 +333,23 @@ int gvv_set_tuning(void* handle, int32_t tile, int32_t large_box) {
   std::lock_guard<std::mutex> lock(h->mu);
   h->tile = tile;
   h->large_box = large_box;
+  return 0;
+}
+
+int gvv_set_depth_pass(void* handle, int32_t mode) {
+  Handle* h = (Handle*)handle;
+  if (!h) return fail("null handle");
+  if (mode != 0 && mode != 1) return fail("depth pass must be 0 (faces binned by size) or 1 (screen tiles)");
+  std::lock_guard<std::mutex> lock(h->mu);
+  h->depth_pass = mode;
+  return 0;
+}
+
+int gvv_get_depth_pass(void* handle, int32_t* mode) {
+  Handle* h = (Handle*)handle;
+  if (!h || !mode) return fail("null pointer");
+  std::lock_guard<std::mutex> lock(h->mu);
+  *mode = h->depth_pass;
   return 0;
 }
 

@@ -15,6 +15,7 @@
 
 #include "gvv_kernels.hpp"
 #include "gvv_raster.hpp"
+#include "gvv_shade.hpp"
 
 namespace gvv {
 
@@ -195,7 +196,8 @@ __device__ __forceinline__ unsigned cand_word(bool packed, bool single, int slot
 // `single`: the candidates of the large-face pass, which stages one face (slot 0).
 __device__ __forceinline__ void flush_candidates(const FaceStage* st, const unsigned* cand, int count,
                                                  const Camera* __restrict__ cams, bool one_view, int bc0,
-                                                 unsigned long long* __restrict__ keys, int W, int H, bool single) {
+                                                 unsigned long long* __restrict__ keys, int W, int H, bool single,
+                                                 TileMarks marks = TileMarks{}) {
   const int lane = threadIdx.x & 63;
   const bool packed = cand_packed(W, H);
   for (int i = lane; i < count; i += 64) {
@@ -219,6 +221,8 @@ __device__ __forceinline__ void flush_candidates(const FaceStage* st, const unsi
     if (!inside(abc)) continue;
     const uint64_t k = pack_key(depth_key(abc, fs.z0, fs.z1, fs.z2), fs.f);
     atomicMin(keys + ((size_t)fs.bc * H + v) * W + u, (unsigned long long)k);
+    // screen-tile depth pass: the tile kernel merges this tile's visibility keys
+    if (marks.dirty) marks.dirty[((size_t)fs.bc * marks.tiles_y + v / kScreenTile) * marks.tiles_x + u / kScreenTile] = 1u;
   }
 }
 
@@ -370,7 +374,7 @@ __device__ __forceinline__ void raster_large_face(const TriSetup* __restrict__ t
                                                   const float4* __restrict__ proj, const Camera* __restrict__ cams,
                                                   unsigned long long* __restrict__ keys, int C, int N, int F, int W,
                                                   int H, int bc, int f, FaceStage* st, unsigned* cand, int* s_xl,
-                                                  int* s_start) {
+                                                  int* s_start, TileMarks marks) {
   const RasterFace r = load_face(faces, proj + (size_t)bc * N, f, W, H);
   const int bw = r.bx.x1 - r.bx.x0 + 1, bh = r.bx.y1 - r.bx.y0 + 1;
   __syncthreads();
@@ -406,7 +410,7 @@ __device__ __forceinline__ void raster_large_face(const TriSetup* __restrict__ t
     for (int q0 = 0; q0 < total; q0 += 64) {
       if (count > kCandCap - 64) {
         __syncthreads();
-        flush_candidates(st, cand, count, cams, true, bc, keys, W, H, true);
+        flush_candidates(st, cand, count, cams, true, bc, keys, W, H, true, marks);
         __syncthreads();
         count = 0;
       }
@@ -433,7 +437,7 @@ __device__ __forceinline__ void raster_large_face(const TriSetup* __restrict__ t
     }
   }
   __syncthreads();
-  flush_candidates(st, cand, count, cams, true, bc, keys, W, H, true);
+  flush_candidates(st, cand, count, cams, true, bc, keys, W, H, true, marks);
 }
 
 // Persistent wavefronts drain the large-face queue: wavefront w rasterises faces w, w + G,
@@ -445,14 +449,14 @@ __global__ void __launch_bounds__(64) k_raster_large(const TriSetup* __restrict_
                                                      const float4* __restrict__ proj, const Camera* __restrict__ cams,
                                                      unsigned long long* __restrict__ keys,
                                                      unsigned int* __restrict__ counters, const int4* __restrict__ large,
-                                                     int C, int N, int F, int W, int H) {
+                                                     int C, int N, int F, int W, int H, TileMarks marks) {
   __shared__ FaceStage st[1];
   __shared__ unsigned cand[kCandCap];
   __shared__ int s_xl[64], s_start[64];
   const unsigned int n_large = counters[0];
   for (unsigned int item = blockIdx.x; item < n_large; item += gridDim.x)
     raster_large_face(tris, faces, proj, cams, keys, C, N, F, W, H, large[item].x, large[item].y, st, cand, s_xl,
-                      s_start);
+                      s_start, marks);
 }
 
 // The same over a queue shared by several meshes: each entry names its mesh, whose raster
@@ -468,7 +472,7 @@ __global__ void __launch_bounds__(64) k_raster_large_multi(const RasterMesh* __r
     const int4 e = large[item];
     const RasterMesh m = meshes[e.z];
     raster_large_face(m.tris, m.faces, m.proj, m.cams, m.keys, m.C, m.N, m.F, m.W, m.H, e.x, e.y, st, cand, s_xl,
-                      s_start);
+                      s_start, m.marks);
   }
 }
 
@@ -491,47 +495,11 @@ __global__ void __launch_bounds__(256) k_resolve(ResolveArgs a) {
   const int64_t i = ((int64_t)bc * a.H + py) * a.W + px;
   unsigned long long key = a.keys[i];
   if (key == kEmptyKey) {
-    a.face[i] = -1;
-    a.bary[2 * i] = 0.f; a.bary[2 * i + 1] = 0.f;
-    a.render[3 * i] = 0.f; a.render[3 * i + 1] = 1.f; a.render[3 * i + 2] = 0.f;
+    background_pixel(a, i);
     return;
   }
   a.keys[i] = kEmptyKey;
-  const int b = bc / a.C;
-  int f = (int)(uint32_t)(key & 0xffffffffu);
-  int i0 = a.faces[3 * f], i1 = a.faces[3 * f + 1], i2 = a.faces[3 * f + 2];
-  // recomputed from the vertices: cheaper here than a 64-byte gather of the face's constants
-  const float* vp = a.vpos + (size_t)b * a.N * 3;
-  const TriSetup t = tri_setup(ld3(vp + 3 * i0), ld3(vp + 3 * i1), ld3(vp + 3 * i2));
-  const Camera& cam = a.cams[bc];
-  V3 d = pixel_ray(cam, px + 0.5f, py + 0.5f);
-  V3 abc = ray_barycentric(t, cam_origin_scaled(cam), d);
-  a.face[i] = f;
-  a.bary[2 * i] = abc.x; a.bary[2 * i + 1] = abc.y;
-
-  const float* vn = a.vnormal + (size_t)bc * a.N * 3;
-  V3 n = pixel_normal(ld3(vn + 3 * i0), ld3(vn + 3 * i1), ld3(vn + 3 * i2), abc);
-  if (dot(n, d) > 0.f) n = -n;
-  V3 col;
-  if (ALBEDO == kTextured) {
-    TexCoord tc = tex_coord(a.uv + 6 * (size_t)f, abc, a.texW, a.texH);
-    int row = min(max(tc.LV, 0), a.texH - 1), cl = min(max(tc.LU, 0), a.texW - 1);
-    col = ld3(a.tex + (size_t)b * a.texH * a.texW * 3 + 3 * ((size_t)a.texW * row + cl));
-  } else if (ALBEDO == kVertexColor) {
-    const float* vc = a.vcol + (size_t)b * a.N * 3;
-    V3 c0 = ld3(vc + 3 * i0), c1 = ld3(vc + 3 * i1), c2 = ld3(vc + 3 * i2);
-    col = v3(c0.x * abc.x + c1.x * abc.y + c2.x * abc.z, c0.y * abc.x + c1.y * abc.y + c2.y * abc.z,
-             c0.z * abc.x + c1.z * abc.y + c2.z * abc.z);
-  } else if (ALBEDO == kNormal) {
-    col = v3((1.f + n.x) / 2.f, (1.f + n.y) / 2.f, (1.f + n.z) / 2.f);
-  } else {
-    col = v3(1.f, 1.f, 1.f);
-  }
-  if (SHADE) {
-    V3 L = sh_illum(n, a.sh + (size_t)bc * 27);
-    col = v3(L.x * col.x, L.y * col.y, L.z * col.z);
-  }
-  a.render[3 * i] = col.x; a.render[3 * i + 1] = col.y; a.render[3 * i + 2] = col.z;
+  shade_pixel<ALBEDO, SHADE>(a, bc, i, (int)(uint32_t)(key & 0xffffffffu), pixel_ray(a.cams[bc], px + 0.5f, py + 0.5f));
 }
 
 __global__ void __launch_bounds__(256) k_normal_map(const UvTexel* __restrict__ table, const int32_t* __restrict__ faces,
@@ -603,6 +571,7 @@ RasterMesh raster_mesh(const ForwardPlan& p) {
   RasterMesh m;
   m.tris = p.tris; m.faces = p.faces; m.proj = p.proj; m.cams = p.cams; m.keys = p.r.keys;
   m.C = p.C; m.N = p.N; m.F = p.F; m.W = p.r.W; m.H = p.r.H;
+  m.marks = p.tiled ? p.marks : TileMarks{};
   return m;
 }
 
@@ -642,7 +611,11 @@ hipError_t launch_forward_raster(const ForwardPlan& p, hipStream_t s) {
                        p.vnormal_out, p.normal_map_out, B, C, N, T);
     return hipGetLastError();
   }
-  {
+  if (p.tiled) {
+    ProfScope ps(p.prof, "k_bin", s);
+    hipError_t e = launch_bin(p, s);
+    if (e != hipSuccess) return e;
+  } else {
     ProfScope ps(p.prof, "k_raster", s);
     hipLaunchKernelGGL(k_raster, dim3(blocks_for((int64_t)B * C * F, 64)), dim3(64), 0, s, p.tris, p.faces, p.proj,
                        p.cams, p.r.keys, p.counters, p.large, B, C, N, F, p.r.W, p.r.H, p.large_box, p.mesh_id);
@@ -650,7 +623,7 @@ hipError_t launch_forward_raster(const ForwardPlan& p, hipStream_t s) {
   if (!p.shared_queue) {
     ProfScope ps(p.prof, "k_raster_large", s);
     hipLaunchKernelGGL(k_raster_large, dim3(kLargeWorkgroups), dim3(64), 0, s, p.tris, p.faces, p.proj, p.cams,
-                       p.r.keys, p.counters, p.large, C, N, F, p.r.W, p.r.H);
+                       p.r.keys, p.counters, p.large, C, N, F, p.r.W, p.r.H, p.tiled ? p.marks : TileMarks{});
   }
   return hipGetLastError();
 }
@@ -658,6 +631,10 @@ hipError_t launch_forward_raster(const ForwardPlan& p, hipStream_t s) {
 hipError_t launch_forward_resolve(const ForwardPlan& p, hipStream_t s) {
   // In normal-map mode the visibility buffer is empty, so this writes the background.
   // One workgroup per 256-pixel row chunk: the count must fit a 32-bit grid dimension.
+  if (p.tiled && !p.normal_map) {
+    ProfScope ps(p.prof, "k_tile", s);
+    return launch_tile(p, s);
+  }
   if ((int64_t)p.r.B * p.r.C * p.r.H * ((p.r.W + 255) / 256) > (int64_t)UINT32_MAX) return hipErrorInvalidConfiguration;
   ProfScope ps(p.prof, "k_resolve", s);
   switch (p.albedo * 2 + (p.shade ? 1 : 0)) {

@@ -32,6 +32,7 @@ struct Handle {
   int device = 0;
   int tile = 16;                    // k_pixel_grad tile edge (8, 16, 32)
   int large_box = 256;              // k_raster: bounding boxes above this go to the large-face queue
+  int depth_pass = 0;               // 0: faces binned by size (k_raster + k_resolve); 1: screen tiles (k_bin + k_tile)
   std::vector<int32_t> faces;
   std::vector<float> uv;
   DevBuf<int32_t> d_faces, d_csr_start, d_csr_count, d_csr_list;
@@ -46,6 +47,7 @@ struct Handle {
   DevBuf<unsigned long long> keys;
   DevBuf<unsigned int> counters;
   DevBuf<int4> large;
+  DevBuf<unsigned> tile_count, tile_dirty, bins;
   DevBuf<float> g_vn;
   DevBuf<float> sh_partial;
   Profiler prof;
@@ -54,7 +56,7 @@ struct Handle {
   ~Handle() {
     d_faces.release(); d_csr_start.release(); d_csr_count.release(); d_csr_list.release();
     d_uv.release(); d_uv_table.release(); cams.release(); fnormal.release(); tris.release(); proj.release();
-    keys.release(); counters.release(); large.release(); g_vn.release(); sh_partial.release();
+    keys.release(); counters.release(); tile_count.release(); tile_dirty.release(); bins.release(); large.release(); g_vn.release(); sh_partial.release();
   }
 };
 

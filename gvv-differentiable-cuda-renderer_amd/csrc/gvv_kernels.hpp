@@ -79,6 +79,17 @@ enum Shading { kShaded = 0, kShadeless = 1 };
 // One entry of the texel -> (face, barycentrics) table used by the normal map.
 struct UvTexel { int32_t face; float a, b, c; };
 
+// Screen tiles of the tile-binned depth pass: kScreenTile x kScreenTile pixels, one workgroup
+// each.  `dirty` flags ([B][C][tiles_y][tiles_x]) the tiles whose pixels the large-face pass
+// wrote into the global visibility keys; null when the face-binned depth pass runs.
+constexpr int kScreenTile = 16;
+constexpr int kBinCap = 512;         // faces per tile bin; a face finding its bin full goes to the large-face pass
+constexpr int kBinMaxTiles = 16;     // faces whose boxes span more tiles go to the large-face pass
+struct TileMarks {
+  unsigned* dirty = nullptr;
+  int tiles_x = 0, tiles_y = 0;
+};
+
 // The raster state of one mesh as the large-face kernel needs it.
 struct RasterMesh {
   const TriSetup* tris;              // [B][F]
@@ -87,6 +98,7 @@ struct RasterMesh {
   const Camera* cams;                // [B][C]
   unsigned long long* keys;          // [B][C][H][W]
   int C, N, F, W, H;
+  TileMarks marks;
 };
 
 struct ResolveArgs {
@@ -130,6 +142,12 @@ struct ForwardPlan {
   bool shared_queue;                 // the queue is shared by several meshes: the caller clears
                                      // the counters and drains it (k_raster_large_multi)
   int large_box;                     // bounding boxes above this many pixels go to the queue
+  // tile-binned depth pass (when tiled): per-tile face bins, their counts and the large-face
+  // marks; counts and marks are zero between calls (the tile kernel resets them)
+  bool tiled;
+  unsigned* tile_count;              // [B][C][tiles_y][tiles_x]
+  unsigned* bins;                    // [B][C][tiles_y][tiles_x][kBinCap] face ids
+  TileMarks marks;
   ResolveArgs r;
   Profiler* prof;
 };
@@ -202,5 +220,9 @@ hipError_t launch_clear_counters(unsigned int* counters, hipStream_t s);
 hipError_t launch_raster_large_multi(const RasterMesh* meshes, unsigned int* counters, const int4* large,
                                      hipStream_t s);
 hipError_t launch_backward(const BackwardPlan& p, hipStream_t s);
+// The tile-binned depth pass (gvv_tile.hip): binning (after k_vertices), and the fused
+// per-tile raster + resolve (after the large-face pass).
+hipError_t launch_bin(const ForwardPlan& p, hipStream_t s);
+hipError_t launch_tile(const ForwardPlan& p, hipStream_t s);
 
 }  // namespace gvv

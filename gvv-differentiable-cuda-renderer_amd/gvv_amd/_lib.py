@@ -15,6 +15,7 @@ LIB_PATH = os.environ.get("GVV_AMD_LIB", os.path.join(_PKG, "libgvv_amd.so"))
 EXPORTS = ("gvv_last_error", "gvv_version", "gvv_create", "gvv_destroy", "gvv_forward", "gvv_backward",
            "gvv_backward_noalloc",
            "gvv_profile_enable", "gvv_profile_report", "gvv_set_tuning", "gvv_get_tuning",
+           "gvv_set_depth_pass", "gvv_get_depth_pass",
            "gvv_calibrate_atomics",
            "gvv_conv_out_size", "gvv_conv_forward", "gvv_conv_backward_data", "gvv_conv_workspace_floats",
            "gvv_conv_backward_weight", "gvv_conv_prepare_weights", "gvv_pack_mosaic",
@@ -60,6 +61,10 @@ def load():
     L.gvv_set_tuning.restype = ctypes.c_int
     L.gvv_get_tuning.argtypes = [P, ctypes.POINTER(i32), ctypes.POINTER(i32)]
     L.gvv_get_tuning.restype = ctypes.c_int
+    L.gvv_set_depth_pass.argtypes = [P, i32]
+    L.gvv_set_depth_pass.restype = ctypes.c_int
+    L.gvv_get_depth_pass.argtypes = [P, ctypes.POINTER(i32)]
+    L.gvv_get_depth_pass.restype = ctypes.c_int
     L.gvv_conv_out_size.argtypes = [i32] * 4
     L.gvv_conv_out_size.restype = ctypes.c_int
     L.gvv_conv_forward.argtypes = [i32] * 8 + [P, P, P, i32, P, i32, P]

@@ -172,6 +172,16 @@ class Rasterizer:
         _lib.check(_lib.load().gvv_get_tuning(self._h, ctypes.byref(t), ctypes.byref(lb)), "gvv_get_tuning")
         return t.value, lb.value
 
+    def set_depth_pass(self, mode: int):
+        """0: faces binned by bounding-box size with a global visibility buffer; 1: screen tiles
+        binned in LDS with the resolve fused at tile end.  Both give identical results."""
+        _lib.check(_lib.load().gvv_set_depth_pass(self._h, int(mode)), "gvv_set_depth_pass")
+
+    def depth_pass(self) -> int:
+        m = ctypes.c_int32()
+        _lib.check(_lib.load().gvv_get_depth_pass(self._h, ctypes.byref(m)), "gvv_get_depth_pass")
+        return m.value
+
     def profile(self, enable: bool = True):
         """Enable/disable per-kernel HIP-event timing of this handle (clears the figures)."""
         _lib.check(_lib.load().gvv_profile_enable(self._h, int(enable)), "gvv_profile_enable")
@@ -240,20 +250,34 @@ def _digest(a):
     return hashlib.blake2b(np.ascontiguousarray(a).tobytes(), digest_size=16).hexdigest()
 
 
+def cache_key(faces, texture_coordinates, number_of_vertices, number_of_cameras, render_resolution_u,
+              render_resolution_v, albedo_mode, shading_mode, image_filter_size, texture_filter_size,
+              compute_normal_map, device_index, stream_id):
+    """Key of the renderer cache: the op attributes (TF caches one op kernel per attr set) plus
+    the device and the HIP stream.  A handle's scratch (visibility keys, cameras, gradient
+    partials) may only be used by one stream at a time, so work queued on two streams -- which
+    the host-side lock of a handle cannot order on the GPU -- gets two handles."""
+    faces = np.asarray(faces, dtype=np.int32).reshape(-1)
+    tc = np.asarray(texture_coordinates, dtype=np.float32).reshape(-1)
+    return (_digest(faces), _digest(tc), int(number_of_vertices), int(number_of_cameras), int(render_resolution_u),
+            int(render_resolution_v), albedo_mode, shading_mode, int(image_filter_size), int(texture_filter_size),
+            bool(compute_normal_map), int(device_index), int(stream_id))
+
+
 def get_rasterizer(faces, texture_coordinates, number_of_vertices, number_of_cameras, render_resolution_u,
                    render_resolution_v, albedo_mode, shading_mode, image_filter_size=2, texture_filter_size=2,
                    compute_normal_map=False) -> Rasterizer:
-    """Returns the cached renderer for these attributes (TF caches one op kernel per attr set)."""
-    faces = np.asarray(faces, dtype=np.int32).reshape(-1)
-    tc = np.asarray(texture_coordinates, dtype=np.float32).reshape(-1)
-    key = (_digest(faces), _digest(tc), int(number_of_vertices), int(number_of_cameras), int(render_resolution_u),
-           int(render_resolution_v), albedo_mode, shading_mode, int(image_filter_size), int(texture_filter_size),
-           bool(compute_normal_map), torch.cuda.current_device())
+    """Returns the cached renderer for these attributes on the current device and stream."""
+    dev = torch.cuda.current_device()
+    key = cache_key(faces, texture_coordinates, number_of_vertices, number_of_cameras, render_resolution_u,
+                    render_resolution_v, albedo_mode, shading_mode, image_filter_size, texture_filter_size,
+                    compute_normal_map, dev, torch.cuda.current_stream(dev).cuda_stream)
     with _cache_lock:
         r = _cache.get(key)
         if r is None:
-            r = Rasterizer(faces, tc, number_of_vertices, number_of_cameras, render_resolution_u, render_resolution_v,
-                           albedo_mode, shading_mode, image_filter_size, texture_filter_size, compute_normal_map)
+            r = Rasterizer(faces, texture_coordinates, number_of_vertices, number_of_cameras, render_resolution_u,
+                           render_resolution_v, albedo_mode, shading_mode, image_filter_size, texture_filter_size,
+                           compute_normal_map)
             if len(_cache) >= _CACHE_MAX:
                 _cache.pop(next(iter(_cache)))
             _cache[key] = r

@@ -111,6 +111,18 @@ int gvv_backward_noalloc(void* handle, int32_t B, int32_t texH, int32_t texW, co
 int gvv_set_tuning(void* handle, int32_t tile, int32_t large_box);
 int gvv_get_tuning(void* handle, int32_t* tile, int32_t* large_box);
 
+/* Depth pass of one handle (no counterpart in the reference, whose two per-(view, face)
+ * bounding-box passes both are replacements for, CUDABasedRasterization.cu:215-408).
+ *   0  faces binned by bounding-box size: wavefront-staged small faces and a persistent
+ *      large-face queue, a 64-bit (depth key | face id) atomicMin into a global visibility
+ *      buffer, then a per-pixel resolve;
+ *   1  16x16-pixel screen tiles: faces binned per tile, each tile rasterised by one workgroup
+ *      against rays computed once per pixel, with its z-buffer in LDS and the resolve and
+ *      shading fused at tile end (faces spanning many tiles still take the large-face queue).
+ * Both give bit-identical outputs. */
+int gvv_set_depth_pass(void* handle, int32_t mode);
+int gvv_get_depth_pass(void* handle, int32_t* mode);
+
 /* Measurement support (no counterpart in the reference): enqueues 2^20 threads x
  * ceil(updates / 2^20) float atomicAdd(+1.0) into buf[0, n_floats) -- scattered (hashed)
  * addresses if `scattered`, else consecutive lanes on consecutive floats -- so that the caller

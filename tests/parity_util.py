@@ -86,7 +86,7 @@ def check_exact(case, face, bary, vnormal, ref):
 
 
 def run_parity(scene, device, albedo, shading, m2d, seed=0, image_fs=1, use_bary=False, check_grads=True, tuning=None,
-               case=None):
+               case=None, depth_pass=None):
     kw = scene.renderer_kwargs()
     kw.update(albedo_mode=albedo, shading_mode=shading)
     case = case or f"{scene.meta.get('name')} {albedo}/{shading}{' m2d' if m2d else ''}"
@@ -96,6 +96,9 @@ def run_parity(scene, device, albedo, shading, m2d, seed=0, image_fs=1, use_bary
     if tuning is not None:
         rast.set_tuning(*tuning)
         assert rast.tuning() == tuple(tuning)
+    if depth_pass is not None:
+        rast.set_depth_pass(depth_pass)
+        assert rast.depth_pass() == depth_pass
     ins = [to_dev(a, device) for a in scene.inputs()]
     bary, face, render, vnormal, _ = rast.forward(*ins)
     torch.cuda.synchronize()

@@ -111,3 +111,17 @@ def test_multi_rejects_modes_without_gradients():
     assert L.gvv_multi_create(d, 1, 2, 0, ctypes.byref(h)) != 0
     assert "vertexColor, textured and foregroundMask" in L.gvv_last_error().decode()
     assert L.gvv_multi_create(d, 0, 0, 0, ctypes.byref(h)) != 0
+
+
+def test_renderer_cache_key_separates_streams_and_devices():
+    """The op-level renderer cache keys on the HIP stream and device as well as the attributes:
+    work on two streams gets two handles (a handle's scratch is for one stream at a time)."""
+    from gvv_amd.renderer import cache_key
+    faces, tc = [0, 1, 2], [0.0] * 6
+    args = (faces, tc, 3, 1, 8, 8, "vertexColor", "shaded", 2, 2, False)
+    k = cache_key(*args, 0, 1234)
+    assert k == cache_key(*args, 0, 1234)
+    assert k != cache_key(*args, 0, 5678)
+    assert k != cache_key(*args, 1, 1234)
+    assert k != cache_key(faces, tc, 3, 1, 8, 8, "textured", "shaded", 2, 2, False, 0, 1234)
+    assert k != cache_key([0, 2, 1], tc, 3, 1, 8, 8, "vertexColor", "shaded", 2, 2, False, 0, 1234)
