@@ -268,7 +268,7 @@ __device__ __forceinline__ void pixel_terms(const BackwardArgs& a, const Camera&
 constexpr int kFaceSlots = 96;
 
 template <int ALBEDO, bool SHADED, bool M2D, bool HAVE_BARY, int TILE>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_pixel_grad(BackwardArgs a) {
+__device__ __forceinline__ void pixel_grad_body(const BackwardArgs& a, int64_t work) {
   static_assert(TILE == 8 || TILE == 16 || TILE == 32, "tile edge must be 8, 16 or 32");
   constexpr int RT = TILE / 8;                            // 8x8 rounds per tile edge
   constexpr bool NEED_POS = SHADED || M2D;
@@ -283,7 +283,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
   __shared__ float s_tval[TS * 3];
   const int lane = threadIdx.x;
   const int tiles = a.tiles_x * a.tiles_y;
-  const int64_t work = xcd_contiguous(blockIdx.x, gridDim.x);    // one (view, tile) per workgroup
+  // one (view, tile) per workgroup
   const int tile = __builtin_amdgcn_readfirstlane((int)(work % tiles));
   const int bc = __builtin_amdgcn_readfirstlane((int)(work / tiles));
   const int b = __builtin_amdgcn_readfirstlane(bc / a.C);
@@ -459,6 +459,20 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
   }
 }
 
+template <int ALBEDO, bool SHADED, bool M2D, bool HAVE_BARY, int TILE>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_pixel_grad(BackwardArgs a) {
+  pixel_grad_body<ALBEDO, SHADED, M2D, HAVE_BARY, TILE>(a, xcd_contiguous(blockIdx.x, gridDim.x));
+}
+
+// The same over several meshes (one tile edge and mode for all of them).  The mesh's arguments
+// are copied from the table first, so they stay in scalar registers across the kernel's stores.
+template <int ALBEDO, bool SHADED, bool M2D, bool HAVE_BARY, int TILE>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_pixel_grad_multi(MeshTable<BackwardArgs> t) {
+  const MeshSlice m = t.find((unsigned)xcd_contiguous(blockIdx.x, gridDim.x));
+  const BackwardArgs a = t.args[m.mesh];
+  pixel_grad_body<ALBEDO, SHADED, M2D, HAVE_BARY, TILE>(a, m.local);
+}
+
 // Fixed-order reduction of the SH tile partials ([b*C + c][27][tiles]): one workgroup per
 // (b, c, coefficient), so the result does not depend on scheduling.
 __device__ __forceinline__ void sh_reduce_block(const float* __restrict__ partial, float* __restrict__ g_sh, int tiles,
@@ -523,13 +537,26 @@ __device__ __forceinline__ void face_jac_block(const float* __restrict__ vpos, c
 
 // The two reductions after the pixel kernel in one launch: blocks [0, fj_blocks) run the
 // face Jacobian, the rest the SH reduction (one per view and coefficient).
-__global__ void __launch_bounds__(256) k_grad_finish(const float* __restrict__ vpos, const int32_t* __restrict__ faces,
-                                                     const float* __restrict__ g_vn, float* __restrict__ g_pos,
-                                                     const float* __restrict__ sh_partial, float* __restrict__ g_sh,
-                                                     int B, int N, int F, int tiles, int fj_blocks) {
+__device__ __forceinline__ void grad_finish_body(const FinishArgs& f, int blk) {
   __shared__ float smem[256 * 13];
-  if ((int)blockIdx.x < fj_blocks) face_jac_block(vpos, faces, g_vn, g_pos, B, N, F, blockIdx.x, smem);
-  else sh_reduce_block(sh_partial, g_sh, tiles, blockIdx.x - fj_blocks, smem);
+  if (blk < f.fj_blocks) face_jac_block(f.vpos, f.faces, f.g_vn, f.g_pos, f.B, f.N, f.F, blk, smem);
+  else sh_reduce_block(f.sh_partial, f.g_sh, f.tiles, blk - f.fj_blocks, smem);
+}
+
+__global__ void __launch_bounds__(256) k_grad_finish(FinishArgs f) { grad_finish_body(f, blockIdx.x); }
+
+__global__ void __launch_bounds__(256) k_grad_finish_multi(MeshTable<FinishArgs> t) {
+  const MeshSlice m = t.find(blockIdx.x);
+  const FinishArgs f = t.args[m.mesh];
+  grad_finish_body(f, (int)m.local);
+}
+
+FinishArgs finish_args(const BackwardArgs& a) {
+  FinishArgs f;
+  f.vpos = a.vpos; f.faces = a.faces; f.g_vn = a.g_vn; f.g_pos = a.g_pos; f.sh_partial = a.sh_partial; f.g_sh = a.g_sh;
+  f.B = a.B; f.N = a.N; f.F = a.F; f.tiles = a.tiles_x * a.tiles_y;
+  f.fj_blocks = (int)(((int64_t)a.B * a.F + 255) / 256);
+  return f;
 }
 
 template <int ALBEDO, bool SHADED, bool M2D, bool HAVE_BARY>
@@ -551,20 +578,55 @@ static void launch_pixel_grad_m(const BackwardArgs& a, hipStream_t s) {
 
 static inline unsigned blocks_for(int64_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
 
+unsigned pixel_grad_blocks(const BackwardArgs& a) { return (unsigned)((int64_t)a.tiles_x * a.tiles_y * a.B * a.C); }
+unsigned finish_blocks(const FinishArgs& f, int C) { return (unsigned)(f.fj_blocks + f.B * C * 27); }
+
+template <int ALBEDO, bool SHADED>
+static void launch_pixel_grad_multi_t(const MeshTable<BackwardArgs>& t, unsigned blocks, int tile, hipStream_t s) {
+  if (tile == 8) hipLaunchKernelGGL((k_pixel_grad_multi<ALBEDO, SHADED, false, false, 8>), dim3(blocks), dim3(64), 0, s, t);
+  else if (tile == 32) hipLaunchKernelGGL((k_pixel_grad_multi<ALBEDO, SHADED, false, false, 32>), dim3(blocks), dim3(64), 0, s, t);
+  else hipLaunchKernelGGL((k_pixel_grad_multi<ALBEDO, SHADED, false, false, 16>), dim3(blocks), dim3(64), 0, s, t);
+}
+
+hipError_t launch_pixel_grad_multi(const MeshTable<BackwardArgs>& t, unsigned blocks, int tile, int albedo, bool shaded,
+                                   hipStream_t s) {
+  if (blocks == 0) return hipSuccess;
+  switch (albedo * 2 + (shaded ? 1 : 0)) {
+    case kVertexColor * 2 + 1: launch_pixel_grad_multi_t<kVertexColor, true>(t, blocks, tile, s); break;
+    case kVertexColor * 2 + 0: launch_pixel_grad_multi_t<kVertexColor, false>(t, blocks, tile, s); break;
+    case kTextured * 2 + 1: launch_pixel_grad_multi_t<kTextured, true>(t, blocks, tile, s); break;
+    case kTextured * 2 + 0: launch_pixel_grad_multi_t<kTextured, false>(t, blocks, tile, s); break;
+    case kForegroundMask * 2 + 1: launch_pixel_grad_multi_t<kForegroundMask, true>(t, blocks, tile, s); break;
+    default: launch_pixel_grad_multi_t<kForegroundMask, false>(t, blocks, tile, s); break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_grad_finish_multi(const MeshTable<FinishArgs>& t, unsigned blocks, hipStream_t s) {
+  if (blocks == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_grad_finish_multi, dim3(blocks), dim3(256), 0, s, t);
+  return hipGetLastError();
+}
+
+SetupArgs backward_setup_args(const BackwardPlan& p) {
+  const BackwardArgs& a = p.a;
+  const int64_t nv = (int64_t)a.B * a.N * 3;
+  SetupArgs sa;
+  sa.extr = p.extr; sa.intr = p.intr; sa.cams = p.cams; sa.n_cams = a.B * a.C;
+  sa.fill[0] = a.g_pos; sa.fill_n[0] = nv;
+  sa.fill[1] = a.g_col; sa.fill_n[1] = nv;
+  sa.fill[2] = a.g_tex; sa.fill_n[2] = (int64_t)a.B * a.texH * a.texW * 3;
+  if (p.shaded) { sa.fill[3] = a.g_vn; sa.fill_n[3] = nv; }
+  else { sa.fill[3] = a.g_sh; sa.fill_n[3] = (int64_t)a.B * a.C * 27; }
+  return sa;
+}
+
 hipError_t launch_backward(const BackwardPlan& p, hipStream_t s) {
   const BackwardArgs& a = p.a;
   {
     // cameras and the zero fills of the gradient outputs in one launch
     ProfScope ps(p.prof, "k_setup", s);
-    const int64_t nv = (int64_t)a.B * a.N * 3;
-    SetupArgs sa;
-    sa.extr = p.extr; sa.intr = p.intr; sa.cams = p.cams; sa.n_cams = a.B * a.C;
-    sa.fill[0] = a.g_pos; sa.fill_n[0] = nv;
-    sa.fill[1] = a.g_col; sa.fill_n[1] = nv;
-    sa.fill[2] = a.g_tex; sa.fill_n[2] = (int64_t)a.B * a.texH * a.texW * 3;
-    if (p.shaded) { sa.fill[3] = a.g_vn; sa.fill_n[3] = nv; }
-    else { sa.fill[3] = a.g_sh; sa.fill_n[3] = (int64_t)a.B * a.C * 27; }
-    hipError_t e = launch_setup(sa, s);
+    hipError_t e = launch_setup(backward_setup_args(p), s);
     if (e != hipSuccess) return e;
   }
   {
@@ -580,9 +642,8 @@ hipError_t launch_backward(const BackwardPlan& p, hipStream_t s) {
   }
   if (p.shaded) {
     ProfScope ps(p.prof, "k_grad_finish", s);
-    const int fj_blocks = (int)blocks_for((int64_t)a.B * a.F);
-    hipLaunchKernelGGL(k_grad_finish, dim3(fj_blocks + a.B * a.C * 27), dim3(256), 0, s, a.vpos, a.faces, a.g_vn,
-                       a.g_pos, a.sh_partial, a.g_sh, a.B, a.N, a.F, a.tiles_x * a.tiles_y, fj_blocks);
+    const FinishArgs f = finish_args(a);
+    hipLaunchKernelGGL(k_grad_finish, dim3(f.fj_blocks + a.B * a.C * 27), dim3(256), 0, s, f);
   }
   return hipGetLastError();
 }

@@ -112,14 +112,18 @@ int plan_forward(Handle* h, int B, int texH, int texW, const float* vpos, const 
   GVV_CHECK(h->large.reserve((size_t)B * h->C * h->F));
   if (h->keys.n != old_keys || old_keys == 0)
     GVV_CHECK(launch_fill_u64(h->keys.p, (int64_t)h->keys.n, kEmptyKey, s));
-  const bool tiled = h->depth_pass == 1;
-  const int tiles_x = tiles(h->W, kScreenTile), tiles_y = tiles(h->H, kScreenTile);
+  const bool tiled = h->depth_pass != 0;
+  // screen tiles: 16 x 16 (mode 1), 32 x 16 (mode 2) or 32 x 32 (mode 3) pixels; bins hold
+  // twice a tile's pixel count
+  const int sx = h->depth_pass >= 2 ? 5 : 4, sy = h->depth_pass == 3 ? 5 : 4;
+  const int tiles_x = tiles(h->W, 1 << sx), tiles_y = tiles(h->H, 1 << sy);
+  const int bin_cap = 2 << (sx + sy);
   if (tiled) {
     const size_t nt = (size_t)B * h->C * tiles_x * tiles_y;
     const size_t old_t = h->tile_count.n;
     GVV_CHECK(h->tile_count.reserve(nt));
     GVV_CHECK(h->tile_dirty.reserve(nt));
-    GVV_CHECK(h->bins.reserve(nt * kBinCap));
+    GVV_CHECK(h->bins.reserve(nt * bin_cap));
     if (h->tile_count.n != old_t || old_t == 0) {
       // counts and marks start at zero; the tile kernel resets the ones it read
       GVV_CHECK(launch_fill_f32(reinterpret_cast<float*>(h->tile_count.p), (int64_t)h->tile_count.n, 0.f, s));
@@ -150,6 +154,9 @@ int plan_forward(Handle* h, int B, int texH, int texW, const float* vpos, const 
   p.marks.dirty = tiled ? h->tile_dirty.p : nullptr;
   p.marks.tiles_x = tiles_x;
   p.marks.tiles_y = tiles_y;
+  p.marks.sx = sx;
+  p.marks.sy = sy;
+  p.marks.bin_cap = bin_cap;
   p.vnormal_out = vnormal;
   p.normal_map_out = normal_map;
   ResolveArgs& r = p.r;
@@ -339,7 +346,8 @@ int gvv_set_tuning(void* handle, int32_t tile, int32_t large_box) {
 int gvv_set_depth_pass(void* handle, int32_t mode) {
   Handle* h = (Handle*)handle;
   if (!h) return fail("null handle");
-  if (mode != 0 && mode != 1) return fail("depth pass must be 0 (faces binned by size) or 1 (screen tiles)");
+  if (mode < 0 || mode > 3)
+    return fail("depth pass must be 0 (faces binned by size) or 1, 2, 3 (screen tiles of 16x16, 32x16, 32x32 px)");
   std::lock_guard<std::mutex> lock(h->mu);
   h->depth_pass = mode;
   return 0;

@@ -22,12 +22,9 @@ namespace gvv {
 // The setup stages of one call as one launch (SetupArgs): blocks [0, cb) set up cameras (block
 // 0 also clears the large-face counters), blocks [cb, cb + fb) set up faces, and the rest
 // zero-fill the listed buffers, 16 bytes per lane where a buffer is 16-byte aligned.
-struct SetupGrid { SetupArgs a; int cam_blocks, face_blocks, fill_blocks; };
-
-__global__ void __launch_bounds__(256) k_setup(SetupGrid g) {
+__device__ __forceinline__ void setup_body(const SetupGrid& g, int blk) {
   const SetupArgs& a = g.a;
   const int t = threadIdx.x;
-  int blk = blockIdx.x;
   if (blk < g.cam_blocks) {
     if (blk == 0 && a.counters && t < 2) a.counters[t] = 0u;
     const int i = blk * 256 + t;
@@ -71,15 +68,29 @@ __global__ void __launch_bounds__(256) k_setup(SetupGrid g) {
   }
 }
 
+__global__ void __launch_bounds__(256) k_setup(SetupGrid g) { setup_body(g, blockIdx.x); }
+
+__global__ void __launch_bounds__(256) k_setup_multi(MeshTable<SetupGrid> t) {
+  const MeshSlice m = t.find(blockIdx.x);
+  const SetupGrid g = t.args[m.mesh];
+  setup_body(g, (int)m.local);
+}
+
 // One thread per (b, c, vertex): vertex normal as the CSR-ordered sum of incident face
 // normals (first term assigned, the rest added), written per view; projection into view c.
-__global__ void __launch_bounds__(256) k_vertices(const float* __restrict__ vpos, const float* __restrict__ fnormal,
-                                                  const int32_t* __restrict__ csr_start, const int32_t* __restrict__ csr_count,
-                                                  const int32_t* __restrict__ csr_list, const Camera* __restrict__ cams,
-                                                  float* __restrict__ vnormal, float4* __restrict__ proj,
-                                                  int B, int C, int N, int F) {
+// `blk` is the mesh's own 256-thread block index.
+__device__ __forceinline__ void vertices_body(const VertexArgs& va, int64_t blk) {
+  const float* __restrict__ vpos = va.vpos;
+  const float* __restrict__ fnormal = va.fnormal;
+  const int32_t* __restrict__ csr_start = va.csr_start;
+  const int32_t* __restrict__ csr_count = va.csr_count;
+  const int32_t* __restrict__ csr_list = va.csr_list;
+  const Camera* __restrict__ cams = va.cams;
+  float* __restrict__ vnormal = va.vnormal;
+  float4* __restrict__ proj = va.proj;
+  const int B = va.B, C = va.C, N = va.N, F = va.F;
   // one thread per (view, vertex)
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = blk * 256 + threadIdx.x;
   const bool valid = i < (int64_t)B * C * N;
   const int v = valid ? (int)(i % N) : 0;
   const int bc = valid ? (int)(i / N) : 0;
@@ -152,6 +163,16 @@ __global__ void __launch_bounds__(256) k_vertices(const float* __restrict__ vpos
   proj[i] = make_float4(p.x, p.y, p.z, 0.f);
 }
 
+__global__ void __launch_bounds__(256) k_vertices(VertexArgs va) { vertices_body(va, blockIdx.x); }
+
+// The multi-mesh kernels copy the mesh's arguments out of the table first, so that they stay in
+// scalar registers across the kernel's stores.
+__global__ void __launch_bounds__(256) k_vertices_multi(MeshTable<VertexArgs> t) {
+  const MeshSlice m = t.find(blockIdx.x);
+  const VertexArgs a = t.args[m.mesh];
+  vertices_body(a, m.local);
+}
+
 constexpr int kLargeWorkgroups = 4096;
 
 struct RasterFace {
@@ -222,7 +243,7 @@ __device__ __forceinline__ void flush_candidates(const FaceStage* st, const unsi
     const uint64_t k = pack_key(depth_key(abc, fs.z0, fs.z1, fs.z2), fs.f);
     atomicMin(keys + ((size_t)fs.bc * H + v) * W + u, (unsigned long long)k);
     // screen-tile depth pass: the tile kernel merges this tile's visibility keys
-    if (marks.dirty) marks.dirty[((size_t)fs.bc * marks.tiles_y + v / kScreenTile) * marks.tiles_x + u / kScreenTile] = 1u;
+    if (marks.dirty) marks.dirty[((size_t)fs.bc * marks.tiles_y + (v >> marks.sy)) * marks.tiles_x + (u >> marks.sx)] = 1u;
   }
 }
 
@@ -276,18 +297,22 @@ __device__ __forceinline__ EdgeTest readlane_edges(const EdgeTest& e, int j) {
 // boxes 17-128 px) 64 was the fastest of 16, 32, 64, 96, 128 and 256.
 constexpr int kTinyBox = 64;
 
-// One wavefront per 64 consecutive (b, c, face) items.  Faces with bounding boxes up to
-// large_box pixels are staged and rasterised here (pre-test, compaction, exact test); larger
-// ones are appended to `large` for k_raster_large.
-__global__ void __launch_bounds__(64) k_raster(const TriSetup* __restrict__ tris, const int32_t* __restrict__ faces,
-                                               const float4* __restrict__ proj, const Camera* __restrict__ cams,
-                                               unsigned long long* __restrict__ keys, unsigned int* __restrict__ large_count,
-                                               int4* __restrict__ large, int B, int C, int N, int F, int W, int H,
-                                               int large_box, int mesh_id) {
+// One wavefront per 64 consecutive (b, c, face) items (wg: the mesh's own logical block).
+// Faces with bounding boxes up to large_box pixels are staged and rasterised here (pre-test,
+// compaction, exact test); larger ones are appended to `large` for k_raster_large.
+__device__ __forceinline__ void raster_body(const RasterArgs& ra, int64_t wg) {
+  const TriSetup* __restrict__ tris = ra.tris;
+  const int32_t* __restrict__ faces = ra.faces;
+  const float4* __restrict__ proj = ra.proj;
+  const Camera* __restrict__ cams = ra.cams;
+  unsigned long long* __restrict__ keys = ra.keys;
+  unsigned int* __restrict__ large_count = ra.large_count;
+  int4* __restrict__ large = ra.large;
+  const int B = ra.B, C = ra.C, N = ra.N, F = ra.F, W = ra.W, H = ra.H, large_box = ra.large_box, mesh_id = ra.mesh_id;
   __shared__ FaceStage st[64];
   __shared__ unsigned cand[kCandCap];
   const int lane = threadIdx.x;
-  const int64_t i = xcd_contiguous(blockIdx.x, gridDim.x) * 64 + lane;
+  const int64_t i = wg * 64 + lane;
   const bool in_range = i < (int64_t)B * C * F;
   const int f = in_range ? (int)(i % F) : 0;
   const int bc = in_range ? (int)(i / F) : 0;
@@ -364,6 +389,14 @@ __global__ void __launch_bounds__(64) k_raster(const TriSetup* __restrict__ tris
   }
   __syncthreads();
   flush_candidates(st, cand, count, cams, one_view, bc0, keys, W, H, false);
+}
+
+__global__ void __launch_bounds__(64) k_raster(RasterArgs ra) { raster_body(ra, xcd_contiguous(blockIdx.x, gridDim.x)); }
+
+__global__ void __launch_bounds__(64) k_raster_multi(MeshTable<RasterArgs> t) {
+  const MeshSlice m = t.find((unsigned)xcd_contiguous(blockIdx.x, gridDim.x));
+  const RasterArgs a = t.args[m.mesh];
+  raster_body(a, m.local);
 }
 
 // One large face (bc, f) rasterised by the whole wavefront.  Instead of the whole bounding box
@@ -481,12 +514,11 @@ __global__ void k_zero_u32(unsigned int* p, int n) {
 }
 
 template <int ALBEDO, bool SHADE>
-__global__ void __launch_bounds__(256) k_resolve(ResolveArgs a) {
+__device__ __forceinline__ void resolve_body(const ResolveArgs& a, unsigned wg) {
   // one workgroup per 256-pixel chunk of one image row: the view, row and chunk follow from the
   // workgroup index alone (uniform arithmetic), so no lane divides its pixel index
   const unsigned chunks = ((unsigned)a.W + 255u) >> 8;
   const unsigned per_view = chunks * (unsigned)a.H;
-  const unsigned wg = (unsigned)xcd_contiguous(blockIdx.x, gridDim.x);
   const int bc = (int)(wg / per_view);
   const unsigned rem = wg - (unsigned)bc * per_view;
   const int py = (int)(rem / chunks);
@@ -500,6 +532,18 @@ __global__ void __launch_bounds__(256) k_resolve(ResolveArgs a) {
   }
   a.keys[i] = kEmptyKey;
   shade_pixel<ALBEDO, SHADE>(a, bc, i, (int)(uint32_t)(key & 0xffffffffu), pixel_ray(a.cams[bc], px + 0.5f, py + 0.5f));
+}
+
+template <int ALBEDO, bool SHADE>
+__global__ void __launch_bounds__(256) k_resolve(ResolveArgs a) {
+  resolve_body<ALBEDO, SHADE>(a, (unsigned)xcd_contiguous(blockIdx.x, gridDim.x));
+}
+
+template <int ALBEDO, bool SHADE>
+__global__ void __launch_bounds__(256) k_resolve_multi(MeshTable<ResolveArgs> t) {
+  const MeshSlice m = t.find((unsigned)xcd_contiguous(blockIdx.x, gridDim.x));
+  const ResolveArgs a = t.args[m.mesh];
+  resolve_body<ALBEDO, SHADE>(a, (unsigned)m.local);
 }
 
 __global__ void __launch_bounds__(256) k_normal_map(const UvTexel* __restrict__ table, const int32_t* __restrict__ faces,
@@ -545,7 +589,7 @@ hipError_t launch_fill_f32(float* p, int64_t n, float v, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_setup(const SetupArgs& a, hipStream_t s) {
+SetupGrid setup_grid(const SetupArgs& a) {
   SetupGrid g;
   g.a = a;
   g.cam_blocks = a.n_cams > 0 || a.counters ? (int)blocks_for(a.n_cams > 0 ? a.n_cams : 1) : 0;
@@ -554,6 +598,11 @@ hipError_t launch_setup(const SetupArgs& a, hipStream_t s) {
   for (int k = 0; k < kFillSegs; ++k)
     if (a.fill[k] && a.fill_n[k] > most) most = a.fill_n[k];
   g.fill_blocks = most > 0 ? (int)std::min<int64_t>(2048, (int64_t)blocks_for((most + 3) / 4)) : 0;
+  return g;
+}
+
+hipError_t launch_setup(const SetupArgs& a, hipStream_t s) {
+  const SetupGrid g = setup_grid(a);
   const int total = g.cam_blocks + g.face_blocks + g.fill_blocks;
   if (total == 0) return hipSuccess;
   hipLaunchKernelGGL(k_setup, dim3(total), dim3(256), 0, s, g);
@@ -565,6 +614,32 @@ static void launch_resolve_t(const ResolveArgs& a, hipStream_t s) {
   const int64_t n = (int64_t)a.B * a.C * a.H * ((a.W + 255) / 256);   // workgroups: row chunks
   if (n <= 0) return;
   hipLaunchKernelGGL((k_resolve<ALBEDO, SHADE>), dim3((unsigned)n), dim3(256), 0, s, a);
+}
+
+SetupArgs forward_setup_args(const ForwardPlan& p) {
+  SetupArgs sa;
+  sa.extr = p.extr; sa.intr = p.intr; sa.cams = p.cams; sa.n_cams = p.B * p.C;
+  sa.vpos = p.vpos; sa.faces = p.faces; sa.fnormal = p.fnormal; sa.tris = p.tris;
+  sa.B = p.B; sa.N = p.N; sa.F = p.F;
+  sa.counters = p.shared_queue ? nullptr : p.counters;
+  return sa;
+}
+
+VertexArgs vertex_args(const ForwardPlan& p) {
+  VertexArgs a;
+  a.vpos = p.vpos; a.fnormal = p.fnormal; a.csr_start = p.csr_start; a.csr_count = p.csr_count;
+  a.csr_list = p.csr_list; a.cams = p.cams; a.vnormal = p.vnormal_out; a.proj = p.proj;
+  a.B = p.B; a.C = p.C; a.N = p.N; a.F = p.F;
+  return a;
+}
+
+RasterArgs raster_args(const ForwardPlan& p) {
+  RasterArgs a;
+  a.tris = p.tris; a.faces = p.faces; a.proj = p.proj; a.cams = p.cams; a.keys = p.r.keys;
+  a.large_count = p.counters; a.large = p.large;
+  a.B = p.B; a.C = p.C; a.N = p.N; a.F = p.F; a.W = p.r.W; a.H = p.r.H;
+  a.large_box = p.large_box; a.mesh_id = p.mesh_id;
+  return a;
 }
 
 RasterMesh raster_mesh(const ForwardPlan& p) {
@@ -591,18 +666,12 @@ hipError_t launch_forward_raster(const ForwardPlan& p, hipStream_t s) {
   {
     // cameras, face setup and (for a queue of its own) the large-face counters
     ProfScope ps(p.prof, "k_setup", s);
-    SetupArgs sa;
-    sa.extr = p.extr; sa.intr = p.intr; sa.cams = p.cams; sa.n_cams = B * C;
-    sa.vpos = p.vpos; sa.faces = p.faces; sa.fnormal = p.fnormal; sa.tris = p.tris;
-    sa.B = B; sa.N = N; sa.F = F;
-    sa.counters = p.shared_queue ? nullptr : p.counters;
-    hipError_t e = launch_setup(sa, s);
+    hipError_t e = launch_setup(forward_setup_args(p), s);
     if (e != hipSuccess) return e;
   }
   {
     ProfScope ps(p.prof, "k_vertices", s);
-    hipLaunchKernelGGL(k_vertices, dim3(blocks_for((int64_t)B * C * N)), dim3(256), 0, s, p.vpos, p.fnormal,
-                       p.csr_start, p.csr_count, p.csr_list, p.cams, p.vnormal_out, p.proj, B, C, N, F);
+    hipLaunchKernelGGL(k_vertices, dim3(blocks_for((int64_t)B * C * N)), dim3(256), 0, s, vertex_args(p));
   }
   if (p.normal_map) {
     ProfScope ps(p.prof, "k_normal_map", s);
@@ -617,8 +686,7 @@ hipError_t launch_forward_raster(const ForwardPlan& p, hipStream_t s) {
     if (e != hipSuccess) return e;
   } else {
     ProfScope ps(p.prof, "k_raster", s);
-    hipLaunchKernelGGL(k_raster, dim3(blocks_for((int64_t)B * C * F, 64)), dim3(64), 0, s, p.tris, p.faces, p.proj,
-                       p.cams, p.r.keys, p.counters, p.large, B, C, N, F, p.r.W, p.r.H, p.large_box, p.mesh_id);
+    hipLaunchKernelGGL(k_raster, dim3(blocks_for((int64_t)B * C * F, 64)), dim3(64), 0, s, raster_args(p));
   }
   if (!p.shared_queue) {
     ProfScope ps(p.prof, "k_raster_large", s);
@@ -651,6 +719,42 @@ hipError_t launch_forward_resolve(const ForwardPlan& p, hipStream_t s) {
   }
   return hipGetLastError();
 }
+
+hipError_t launch_setup_multi(const MeshTable<SetupGrid>& t, unsigned blocks, hipStream_t s) {
+  if (blocks == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_setup_multi, dim3(blocks), dim3(256), 0, s, t);
+  return hipGetLastError();
+}
+
+hipError_t launch_vertices_multi(const MeshTable<VertexArgs>& t, unsigned blocks, hipStream_t s) {
+  if (blocks == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_vertices_multi, dim3(blocks), dim3(256), 0, s, t);
+  return hipGetLastError();
+}
+
+hipError_t launch_raster_multi(const MeshTable<RasterArgs>& t, unsigned blocks, hipStream_t s) {
+  if (blocks == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_raster_multi, dim3(blocks), dim3(64), 0, s, t);
+  return hipGetLastError();
+}
+
+hipError_t launch_resolve_multi(const MeshTable<ResolveArgs>& t, unsigned blocks, int albedo, bool shade,
+                                hipStream_t s) {
+  if (blocks == 0) return hipSuccess;
+  switch (albedo * 2 + (shade ? 1 : 0)) {
+    case kVertexColor * 2 + 0: hipLaunchKernelGGL((k_resolve_multi<kVertexColor, false>), dim3(blocks), dim3(256), 0, s, t); break;
+    case kVertexColor * 2 + 1: hipLaunchKernelGGL((k_resolve_multi<kVertexColor, true>), dim3(blocks), dim3(256), 0, s, t); break;
+    case kTextured * 2 + 0: hipLaunchKernelGGL((k_resolve_multi<kTextured, false>), dim3(blocks), dim3(256), 0, s, t); break;
+    case kTextured * 2 + 1: hipLaunchKernelGGL((k_resolve_multi<kTextured, true>), dim3(blocks), dim3(256), 0, s, t); break;
+    case kForegroundMask * 2 + 0:
+      hipLaunchKernelGGL((k_resolve_multi<kForegroundMask, false>), dim3(blocks), dim3(256), 0, s, t);
+      break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+unsigned resolve_blocks(const ResolveArgs& a) { return (unsigned)((int64_t)a.B * a.C * a.H * ((a.W + 255) / 256)); }
 
 hipError_t launch_forward(const ForwardPlan& p, hipStream_t s) {
   hipError_t e = launch_forward_raster(p, s);

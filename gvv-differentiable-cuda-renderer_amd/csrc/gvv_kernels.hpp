@@ -71,6 +71,29 @@ __device__ __forceinline__ int64_t xcd_contiguous(int64_t wg, int64_t n) {
   return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + j;
 }
 
+// One launch over several meshes: mesh i owns the blocks [start[i], start[i+1]) of the grid
+// (in the logical order, after any XCD remapping) and takes args[i].  find() is a binary search
+// over the uniform prefix table.
+struct MeshSlice { int mesh; int64_t local; };
+template <class Args>
+struct MeshTable {
+  const Args* args;                  // [n]
+  const unsigned* start;             // [n + 1]
+  int n;
+  __device__ __forceinline__ MeshSlice find(unsigned wg) const {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (start[mid] <= wg) lo = mid;
+      else hi = mid - 1;
+    }
+    MeshSlice r;
+    r.mesh = lo;
+    r.local = (int64_t)(wg - start[lo]);
+    return r;
+  }
+};
+
 // Albedo and shading modes, numbered as the reference enums
 // (cpp/src/Renderer/CUDABasedRasterizationInput.h:25-35).
 enum Albedo { kVertexColor = 0, kTextured = 1, kNormal = 2, kLighting = 3, kForegroundMask = 4 };
@@ -79,15 +102,16 @@ enum Shading { kShaded = 0, kShadeless = 1 };
 // One entry of the texel -> (face, barycentrics) table used by the normal map.
 struct UvTexel { int32_t face; float a, b, c; };
 
-// Screen tiles of the tile-binned depth pass: kScreenTile x kScreenTile pixels, one workgroup
-// each.  `dirty` flags ([B][C][tiles_y][tiles_x]) the tiles whose pixels the large-face pass
-// wrote into the global visibility keys; null when the face-binned depth pass runs.
-constexpr int kScreenTile = 16;
-constexpr int kBinCap = 512;         // faces per tile bin; a face finding its bin full goes to the large-face pass
+// Screen tiles of the tile-binned depth pass: (1 << sx) x (1 << sy) pixels, one workgroup
+// each, with a bin of bin_cap face ids.  `dirty` flags ([B][C][tiles_y][tiles_x]) the tiles
+// whose pixels the large-face pass wrote into the global visibility keys; null when the
+// face-binned depth pass runs.
 constexpr int kBinMaxTiles = 16;     // faces whose boxes span more tiles go to the large-face pass
 struct TileMarks {
   unsigned* dirty = nullptr;
   int tiles_x = 0, tiles_y = 0;
+  int sx = 4, sy = 4;                // log2 of the tile width and height
+  int bin_cap = 512;                 // faces per tile bin (a face finding its bin full goes to the large-face pass)
 };
 
 // The raster state of one mesh as the large-face kernel needs it.
@@ -99,6 +123,29 @@ struct RasterMesh {
   unsigned long long* keys;          // [B][C][H][W]
   int C, N, F, W, H;
   TileMarks marks;
+};
+
+struct VertexArgs {
+  const float* vpos;                 // [B][N][3]
+  const float* fnormal;              // [B][F][3]
+  const int32_t* csr_start;
+  const int32_t* csr_count;
+  const int32_t* csr_list;
+  const Camera* cams;                // [B][C]
+  float* vnormal;                    // [B][C][N][3]
+  float4* proj;                      // [B][C][N]
+  int B, C, N, F;
+};
+
+struct RasterArgs {
+  const TriSetup* tris;              // [B][F]
+  const int32_t* faces;              // [F][3]
+  const float4* proj;                // [B][C][N]
+  const Camera* cams;                // [B][C]
+  unsigned long long* keys;          // [B][C][H][W]
+  unsigned int* large_count;         // queue length
+  int4* large;                       // large-face queue
+  int B, C, N, F, W, H, large_box, mesh_id;
 };
 
 struct ResolveArgs {
@@ -146,7 +193,7 @@ struct ForwardPlan {
   // marks; counts and marks are zero between calls (the tile kernel resets them)
   bool tiled;
   unsigned* tile_count;              // [B][C][tiles_y][tiles_x]
-  unsigned* bins;                    // [B][C][tiles_y][tiles_x][kBinCap] face ids
+  unsigned* bins;                    // [B][C][tiles_y][tiles_x][bin_cap] face ids
   TileMarks marks;
   ResolveArgs r;
   Profiler* prof;
@@ -176,6 +223,19 @@ struct BackwardArgs {
   float* sh_partial;                 // [B*C][tiles][27] scratch
   float* g_sh;                       // [B][C][27]
 };
+
+// The reductions after the pixel kernel (k_grad_finish): fj_blocks face-Jacobian blocks, then
+// one SH block per (view, coefficient).
+struct FinishArgs {
+  const float* vpos;
+  const int32_t* faces;
+  const float* g_vn;
+  float* g_pos;
+  const float* sh_partial;
+  float* g_sh;
+  int B, N, F, tiles, fj_blocks;
+};
+FinishArgs finish_args(const BackwardArgs& a);
 
 struct BackwardPlan {
   int albedo;
@@ -208,6 +268,9 @@ struct SetupArgs {
   int64_t fill_n[kFillSegs] = {};
   unsigned int* counters = nullptr;
 };
+// The block layout of one setup launch (k_setup): camera, face and fill blocks.
+struct SetupGrid { SetupArgs a; int cam_blocks, face_blocks, fill_blocks; };
+SetupGrid setup_grid(const SetupArgs& a);
 hipError_t launch_setup(const SetupArgs& a, hipStream_t s);
 hipError_t launch_forward(const ForwardPlan& p, hipStream_t s);
 // The two halves of launch_forward: up to and including the depth pass (with the large-face
@@ -215,11 +278,29 @@ hipError_t launch_forward(const ForwardPlan& p, hipStream_t s);
 hipError_t launch_forward_raster(const ForwardPlan& p, hipStream_t s);
 hipError_t launch_forward_resolve(const ForwardPlan& p, hipStream_t s);
 RasterMesh raster_mesh(const ForwardPlan& p);
+SetupArgs forward_setup_args(const ForwardPlan& p);
+VertexArgs vertex_args(const ForwardPlan& p);
+RasterArgs raster_args(const ForwardPlan& p);
 hipError_t launch_clear_counters(unsigned int* counters, hipStream_t s);
 // Drains a large-face queue shared by several meshes (entries carry the mesh index).
 hipError_t launch_raster_large_multi(const RasterMesh* meshes, unsigned int* counters, const int4* large,
                                      hipStream_t s);
 hipError_t launch_backward(const BackwardPlan& p, hipStream_t s);
+
+// Stage launches over several meshes (the multi-mesh runtime): one launch per stage, each
+// workgroup finding its mesh in the table.  `blocks` is the table's total block count.
+SetupArgs backward_setup_args(const BackwardPlan& p);
+unsigned resolve_blocks(const ResolveArgs& a);
+unsigned pixel_grad_blocks(const BackwardArgs& a);
+unsigned finish_blocks(const FinishArgs& f, int C);
+hipError_t launch_setup_multi(const MeshTable<SetupGrid>& t, unsigned blocks, hipStream_t s);
+hipError_t launch_vertices_multi(const MeshTable<VertexArgs>& t, unsigned blocks, hipStream_t s);
+hipError_t launch_raster_multi(const MeshTable<RasterArgs>& t, unsigned blocks, hipStream_t s);
+hipError_t launch_resolve_multi(const MeshTable<ResolveArgs>& t, unsigned blocks, int albedo, bool shade,
+                                hipStream_t s);
+hipError_t launch_pixel_grad_multi(const MeshTable<BackwardArgs>& t, unsigned blocks, int tile, int albedo, bool shaded,
+                                   hipStream_t s);
+hipError_t launch_grad_finish_multi(const MeshTable<FinishArgs>& t, unsigned blocks, hipStream_t s);
 // The tile-binned depth pass (gvv_tile.hip): binning (after k_vertices), and the fused
 // per-tile raster + resolve (after the large-face pass).
 hipError_t launch_bin(const ForwardPlan& p, hipStream_t s);

@@ -51,11 +51,34 @@ int size_class(const Handle* h) {
   return tb * 3 + pb;
 }
 
+// The per-stage argument tables of a batched run (one launch per stage across all meshes):
+// for each stage the meshes' argument records and the prefix of their block counts, packed
+// into one device buffer (uploaded when it changes).
+enum Stage { kFwdSetup, kVertices, kRaster, kResolve, kBwdSetup, kPixelGrad8, kPixelGrad16, kPixelGrad32, kFinish,
+             kStages };
+struct StageTables {
+  std::vector<unsigned char> host;
+  DevBuf<unsigned char> dev;
+  size_t off_args[kStages] = {}, off_start[kStages] = {};
+  int n[kStages] = {};
+  unsigned blocks[kStages] = {};
+  template <class A>
+  MeshTable<A> table(int st) const {
+    MeshTable<A> t;
+    t.args = reinterpret_cast<const A*>(dev.p + off_args[st]);
+    t.start = reinterpret_cast<const unsigned*>(dev.p + off_start[st]);
+    t.n = n[st];
+    return t;
+  }
+};
+
 struct Multi {
   std::vector<Handle*> meshes;
   std::vector<int> cls;
   int albedo = 0, shading = 0, device = 0;
   bool use_graph = true;
+  bool batched = false;                         // one launch per stage across meshes
+  StageTables stages;
   hipStream_t lanes[kLanes] = {};
   hipEvent_t fork[2] = {};
   hipEvent_t join[2][kLanes] = {};
@@ -88,6 +111,7 @@ struct Multi {
     counters.release();
     large.release();
     table.release();
+    stages.dev.release();
     for (auto* h : meshes) gvv_destroy(h);
   }
 };
@@ -130,6 +154,103 @@ int check_device(const Multi* m) {
   return 0;
 }
 
+template <class A>
+void put_stage(StageTables& st, std::vector<unsigned char>& blob, int stage, const std::vector<A>& args,
+               const std::vector<unsigned>& blocks) {
+  auto align = [&](size_t a) { blob.resize((blob.size() + a - 1) / a * a); };
+  align(256);
+  st.off_args[stage] = blob.size();
+  const unsigned char* a = reinterpret_cast<const unsigned char*>(args.data());
+  blob.insert(blob.end(), a, a + sizeof(A) * args.size());
+  align(256);
+  st.off_start[stage] = blob.size();
+  std::vector<unsigned> start(args.size() + 1, 0u);
+  for (size_t i = 0; i < args.size(); ++i) start[i + 1] = start[i] + blocks[i];
+  const unsigned char* b = reinterpret_cast<const unsigned char*>(start.data());
+  blob.insert(blob.end(), b, b + sizeof(unsigned) * start.size());
+  st.n[stage] = (int)args.size();
+  st.blocks[stage] = start.back();
+}
+
+// The per-stage tables of a batched run from the meshes' plans (uploaded on `s` when changed).
+int build_stages(Multi* m, const std::vector<ForwardPlan>& fp, const std::vector<BackwardPlan>& bp, bool backward,
+                 hipStream_t s) {
+  const int n = (int)fp.size();
+  StageTables& st = m->stages;
+  std::vector<unsigned char> blob;
+  std::vector<SetupGrid> setup(n);
+  std::vector<VertexArgs> vert(n);
+  std::vector<RasterArgs> rast(n);
+  std::vector<ResolveArgs> res(n);
+  std::vector<unsigned> b_setup(n), b_vert(n), b_rast(n), b_res(n);
+  for (int i = 0; i < n; ++i) {
+    setup[i] = setup_grid(forward_setup_args(fp[i]));
+    b_setup[i] = (unsigned)(setup[i].cam_blocks + setup[i].face_blocks + setup[i].fill_blocks);
+    vert[i] = vertex_args(fp[i]);
+    b_vert[i] = (unsigned)(((int64_t)fp[i].B * fp[i].C * fp[i].N + 255) / 256);
+    rast[i] = raster_args(fp[i]);
+    b_rast[i] = (unsigned)(((int64_t)fp[i].B * fp[i].C * fp[i].F + 63) / 64);
+    res[i] = fp[i].r;
+    b_res[i] = resolve_blocks(fp[i].r);
+  }
+  put_stage(st, blob, kFwdSetup, setup, b_setup);
+  put_stage(st, blob, kVertices, vert, b_vert);
+  put_stage(st, blob, kRaster, rast, b_rast);
+  put_stage(st, blob, kResolve, res, b_res);
+  for (int k = kBwdSetup; k < kStages; ++k) { st.n[k] = 0; st.blocks[k] = 0; }
+  if (backward) {
+    std::vector<SetupGrid> bsetup(n);
+    std::vector<unsigned> b_bsetup(n);
+    std::vector<BackwardArgs> grad[3];
+    std::vector<unsigned> b_grad[3];
+    std::vector<FinishArgs> fin;
+    std::vector<unsigned> b_fin;
+    for (int i = 0; i < n; ++i) {
+      bsetup[i] = setup_grid(backward_setup_args(bp[i]));
+      b_bsetup[i] = (unsigned)(bsetup[i].cam_blocks + bsetup[i].face_blocks + bsetup[i].fill_blocks);
+      const int g = bp[i].a.tile == 8 ? 0 : bp[i].a.tile == 16 ? 1 : 2;
+      grad[g].push_back(bp[i].a);
+      b_grad[g].push_back(pixel_grad_blocks(bp[i].a));
+      if (bp[i].shaded) {
+        fin.push_back(finish_args(bp[i].a));
+        b_fin.push_back(finish_blocks(fin.back(), bp[i].a.C));
+      }
+    }
+    put_stage(st, blob, kBwdSetup, bsetup, b_bsetup);
+    for (int g = 0; g < 3; ++g) put_stage(st, blob, kPixelGrad8 + g, grad[g], b_grad[g]);
+    put_stage(st, blob, kFinish, fin, b_fin);
+  }
+  if (blob != st.host || !st.dev.p) {
+    // the previous contents may still be read by queued work; the copy source must outlive it
+    MULTI_CHECK(hipStreamSynchronize(s));
+    MULTI_CHECK(st.dev.reserve(blob.size()));
+    st.host = std::move(blob);
+    MULTI_CHECK(hipMemcpyAsync(st.dev.p, st.host.data(), st.host.size(), hipMemcpyHostToDevice, s));
+  }
+  return 0;
+}
+
+// One launch per stage across all meshes: clear the queue, setup, vertices, depth pass, the
+// shared large-face launch, resolve; then the backward's setup, pixel kernels (one launch per
+// tile edge in use) and finishing reductions.
+hipError_t enqueue_batched(Multi* m, bool backward, hipStream_t s) {
+  const StageTables& st = m->stages;
+  const bool shade = (m->shading == kShaded) && m->albedo != kNormal;
+  hipError_t e = launch_clear_counters(m->counters.p, s);
+  if (e == hipSuccess) e = launch_setup_multi(st.table<SetupGrid>(kFwdSetup), st.blocks[kFwdSetup], s);
+  if (e == hipSuccess) e = launch_vertices_multi(st.table<VertexArgs>(kVertices), st.blocks[kVertices], s);
+  if (e == hipSuccess) e = launch_raster_multi(st.table<RasterArgs>(kRaster), st.blocks[kRaster], s);
+  if (e == hipSuccess) e = launch_raster_large_multi(m->table.p, m->counters.p, m->large.p, s);
+  if (e == hipSuccess) e = launch_resolve_multi(st.table<ResolveArgs>(kResolve), st.blocks[kResolve], m->albedo, shade, s);
+  if (!backward) return e;
+  if (e == hipSuccess) e = launch_setup_multi(st.table<SetupGrid>(kBwdSetup), st.blocks[kBwdSetup], s);
+  for (int g = 0; g < 3 && e == hipSuccess; ++g)
+    e = launch_pixel_grad_multi(st.table<BackwardArgs>(kPixelGrad8 + g), st.blocks[kPixelGrad8 + g], 8 << g, m->albedo,
+                                m->shading == kShaded, s);
+  if (e == hipSuccess) e = launch_grad_finish_multi(st.table<FinishArgs>(kFinish), st.blocks[kFinish], s);
+  return e;
+}
+
 // Plans every mesh onto the shared large-face queue and brings the device copy of the mesh
 // table up to date (on `s`, after the work already queued there, which may still read it).
 int plan_all(Multi* m, const gvv_mesh_io* io, bool backward, hipStream_t s, std::vector<ForwardPlan>& fp,
@@ -162,6 +283,7 @@ int plan_all(Multi* m, const gvv_mesh_io* io, bool backward, hipStream_t s, std:
     MULTI_CHECK(hipMemcpyAsync(m->table.p, m->host_table.data(), t.size() * sizeof(RasterMesh),
                                hipMemcpyHostToDevice, s));
   }
+  if (m->batched) return build_stages(m, fp, bp, backward, s);
   return 0;
 }
 
@@ -187,9 +309,13 @@ std::vector<char> graph_key(const Multi* m, const gvv_mesh_io* io, bool backward
     memcpy(t + 8 * i, &m->meshes[i]->tile, sizeof(int));
     memcpy(t + 8 * i + 4, &m->meshes[i]->large_box, sizeof(int));
   }
-  // the shared queue's buffers are baked into the graph
+  // the shared queue's buffers and the stage tables are baked into the graph
   const void* bufs[3] = {m->counters.p, m->large.p, m->table.p};
   memcpy(t + 8 * n, bufs, sizeof(bufs));
+  const void* stage_buf = m->batched ? (const void*)m->stages.dev.p : nullptr;
+  const char* sb = reinterpret_cast<const char*>(&stage_buf);
+  k.insert(k.end(), sb, sb + sizeof(stage_buf));
+  k.push_back(m->batched ? 1 : 0);
   return k;
 }
 
@@ -227,12 +353,17 @@ int capture(Multi* m, const gvv_mesh_io* io, bool backward, const std::vector<Fo
   });
   m->drop_graph();
   MULTI_CHECK(hipStreamBeginCapture(m->lanes[0], hipStreamCaptureModeThreadLocal));
-  hipError_t e = launch_clear_counters(m->counters.p, m->lanes[0]);
-  if (e == hipSuccess)
-    e = fan_out(m, order, io, 0, [&](int i, hipStream_t s) { return launch_forward_raster(fp[i], s); });
-  if (e == hipSuccess) e = launch_raster_large_multi(m->table.p, m->counters.p, m->large.p, m->lanes[0]);
-  if (e == hipSuccess)
-    e = fan_out(m, order, io, 1, [&](int i, hipStream_t s) { return enqueue_tail(fp[i], bp[i], backward, s); });
+  hipError_t e = hipSuccess;
+  if (m->batched) {
+    e = enqueue_batched(m, backward, m->lanes[0]);
+  } else {
+    e = launch_clear_counters(m->counters.p, m->lanes[0]);
+    if (e == hipSuccess)
+      e = fan_out(m, order, io, 0, [&](int i, hipStream_t s) { return launch_forward_raster(fp[i], s); });
+    if (e == hipSuccess) e = launch_raster_large_multi(m->table.p, m->counters.p, m->large.p, m->lanes[0]);
+    if (e == hipSuccess)
+      e = fan_out(m, order, io, 1, [&](int i, hipStream_t s) { return enqueue_tail(fp[i], bp[i], backward, s); });
+  }
   hipGraph_t g = nullptr;
   const hipError_t e_end = hipStreamEndCapture(m->lanes[0], &g);
   if (e != hipSuccess || e_end != hipSuccess) {
@@ -296,6 +427,15 @@ int gvv_multi_use_graph(void* handle, int32_t on) {
   return 0;
 }
 
+int gvv_multi_set_batched(void* handle, int32_t on) {
+  Multi* m = (Multi*)handle;
+  if (!m) return fail("null handle");
+  std::lock_guard<std::mutex> lock(m->mu);
+  m->batched = on != 0;
+  m->drop_graph();
+  return 0;
+}
+
 int gvv_multi_run(void* handle, const gvv_mesh_io* io, int32_t backward, void* stream) {
   Multi* m = (Multi*)handle;
   if (!m) return fail("null handle");
@@ -307,7 +447,8 @@ int gvv_multi_run(void* handle, const gvv_mesh_io* io, int32_t backward, void* s
   std::vector<BackwardPlan> bp;
   if (plan_all(m, io, backward != 0, s, fp, bp)) return 1;
   if (!m->use_graph) {
-    MULTI_CHECK(enqueue_all(m, fp, bp, backward != 0, s));
+    if (m->batched) MULTI_CHECK(enqueue_batched(m, backward != 0, s));
+    else MULTI_CHECK(enqueue_all(m, fp, bp, backward != 0, s));
     return 0;
   }
   if (!m->exec || m->key != graph_key(m, io, backward != 0))
@@ -366,6 +507,16 @@ int gvv_multi_autotune(void* handle, const gvv_mesh_io* io, int32_t reps, void* 
   (void)hipEventDestroy(b);
   m->drop_graph();
   return rc;
+}
+
+int gvv_multi_set_tuning(void* handle, int32_t mesh, int32_t tile, int32_t large_box) {
+  Multi* m = (Multi*)handle;
+  if (!m) return fail("null handle");
+  std::lock_guard<std::mutex> lock(m->mu);
+  if (mesh < 0 || mesh >= (int32_t)m->meshes.size()) return fail("mesh index out of range");
+  if (gvv_set_tuning(m->meshes[mesh], tile, large_box)) return 1;
+  m->drop_graph();
+  return 0;
 }
 
 int gvv_multi_tuning(void* handle, int32_t mesh, int32_t* tile, int32_t* large_box, int32_t* size_class) {

@@ -20,6 +20,7 @@ EXPORTS = ("gvv_last_error", "gvv_version", "gvv_create", "gvv_destroy", "gvv_fo
            "gvv_conv_out_size", "gvv_conv_forward", "gvv_conv_backward_data", "gvv_conv_workspace_floats",
            "gvv_conv_backward_weight", "gvv_conv_prepare_weights", "gvv_pack_mosaic",
            "gvv_multi_create", "gvv_multi_destroy", "gvv_multi_run", "gvv_multi_use_graph", "gvv_multi_autotune",
+           "gvv_multi_set_batched", "gvv_multi_set_tuning",
            "gvv_multi_tuning")
 
 _lib = None
@@ -87,6 +88,10 @@ def load():
     L.gvv_multi_run.restype = ctypes.c_int
     L.gvv_multi_use_graph.argtypes = [P, i32]
     L.gvv_multi_use_graph.restype = ctypes.c_int
+    L.gvv_multi_set_batched.argtypes = [P, i32]
+    L.gvv_multi_set_batched.restype = ctypes.c_int
+    L.gvv_multi_set_tuning.argtypes = [P, i32, i32, i32]
+    L.gvv_multi_set_tuning.restype = ctypes.c_int
     L.gvv_multi_autotune.argtypes = [P, P, i32, P]
     L.gvv_multi_autotune.restype = ctypes.c_int
     L.gvv_multi_tuning.argtypes = [P, i32, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32)]

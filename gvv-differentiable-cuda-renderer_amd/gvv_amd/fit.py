@@ -47,6 +47,7 @@ class FitConfig:
     lr: float = 1e-2
     seed: int = 0
     colour_smoothing: int = 0     # Laplacian steps applied to the template's random vertex colours
+    basis_smoothing: int = 60     # Laplacian steps applied to the Gaussian displacement fields (0: white)
 
 
 def _smooth(faces, n, X, iters):
@@ -112,7 +113,7 @@ class ShapeModel:
         self.faces = f
         self.uv = uv
         self.N, self.F = v.shape[0], f.shape[0]
-        B = _smooth_fields(r_basis, f, self.N, cfg.n_basis).reshape(cfg.n_basis, -1).T     # (3N, K)
+        B = _smooth_fields(r_basis, f, self.N, cfg.n_basis, cfg.basis_smoothing).reshape(cfg.n_basis, -1).T  # (3N, K)
         Q, _ = np.linalg.qr(B)
         Q = Q.T.reshape(cfg.n_basis, self.N, 3) * (cfg.basis_scale_m * MM * math.sqrt(self.N))
         dev = self.device

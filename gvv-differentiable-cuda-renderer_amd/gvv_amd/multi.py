@@ -8,6 +8,9 @@ counterpart there.
 from __future__ import annotations
 
 import ctypes
+import json
+import os
+import tempfile
 
 import numpy as np
 import torch
@@ -34,6 +37,33 @@ def _p(t):
     return t.data_ptr() if t is not None else None
 
 
+def device_key(device) -> str:
+    """Autotuner cache key of a GPU: product name, architecture and compute-unit count."""
+    props = torch.cuda.get_device_properties(device)
+    return f"{props.name}|{getattr(props, 'gcnArchName', '')}|{props.multi_processor_count}"
+
+
+def _read_cache(path: str) -> dict:
+    """The autotuner cache file: {device key: {"albedo/shading": {size class: {"tile", "large_box"}}}}."""
+    try:
+        with open(path) as fh:
+            data = json.load(fh)
+    except FileNotFoundError:
+        return {}
+    if not isinstance(data, dict) or data.get("version") != 1:
+        return {}
+    return data.get("devices", {})
+
+
+def _write_cache(path: str, devices: dict):
+    d = os.path.dirname(os.path.abspath(path))
+    os.makedirs(d, exist_ok=True)
+    fd, tmp = tempfile.mkstemp(dir=d, prefix=".autotune", suffix=".json")
+    with os.fdopen(fd, "w") as fh:
+        json.dump({"version": 1, "devices": devices}, fh, indent=1, sort_keys=True)
+    os.replace(tmp, path)
+
+
 class MultiRenderer:
     """Owns the runtime handle and, per mesh, device copies of the scene inputs plus output and
     gradient buffers.  `scenes` are gvv_amd.scenes.Scene objects (one mesh each, any batch)."""
@@ -41,6 +71,7 @@ class MultiRenderer:
     def __init__(self, scenes, albedo_mode="vertexColor", shading_mode="shaded", device=None):
         self.device = device or torch.device("cuda", torch.cuda.current_device())
         self.scenes = list(scenes)
+        self.albedo_mode, self.shading_mode = albedo_mode, shading_mode
         L = _lib.load()
         self._keep = []
         descs = (MeshDesc * len(self.scenes))()
@@ -90,12 +121,43 @@ class MultiRenderer:
     def use_graph(self, on: bool):
         _lib.check(_lib.load().gvv_multi_use_graph(self._h, int(on)), "gvv_multi_use_graph")
 
+    def set_batched(self, on: bool):
+        """True: one launch per stage across all meshes; False (default): per-mesh launches on
+        parallel graph branches."""
+        _lib.check(_lib.load().gvv_multi_set_batched(self._h, int(on)), "gvv_multi_set_batched")
+
     def run(self, backward: bool = True):
         """Forward of every mesh (+ backward against each mesh's bufs['g_render'])."""
         _lib.check(_lib.load().gvv_multi_run(self._h, self.io, int(backward), self._stream()), "gvv_multi_run")
 
-    def autotune(self, reps: int = 3):
+    def autotune(self, reps: int = 3, cache: str | None = None) -> str:
+        """Chooses the launch settings per size class (gvv_multi_autotune).  With `cache`, the path
+        of a JSON file of earlier choices keyed by device and mode: when every size class present
+        has an entry there it is applied without timing anything ("cache"); otherwise the
+        autotuner times the classes ("timed") and the file is updated with its choices."""
+        n = len(self.scenes)
+        classes = [self.tuning(i)[2] for i in range(n)]
+        dev_key = device_key(self.device)
+        mode_key = f"{self.albedo_mode}/{self.shading_mode}"
+        data = _read_cache(cache) if cache else {}
+        entries = data.get(dev_key, {}).get(mode_key, {})
+        if cache and all(str(c) in entries for c in set(classes)):
+            for i, c in enumerate(classes):
+                e = entries[str(c)]
+                self.set_tuning(i, e["tile"], e["large_box"])
+            return "cache"
         _lib.check(_lib.load().gvv_multi_autotune(self._h, self.io, int(reps), self._stream()), "gvv_multi_autotune")
+        if cache:
+            for i, c in enumerate(classes):
+                tile, box, _ = self.tuning(i)
+                entries[str(c)] = {"tile": tile, "large_box": box}
+            data.setdefault(dev_key, {})[mode_key] = entries
+            _write_cache(cache, data)
+        return "timed"
+
+    def set_tuning(self, i: int, tile: int, large_box: int):
+        """Backward tile edge and large-face threshold of mesh i."""
+        _lib.check(_lib.load().gvv_multi_set_tuning(self._h, int(i), int(tile), int(large_box)), "gvv_multi_set_tuning")
 
     def tuning(self, i: int):
         """(tile, large_box, size_class) of mesh i."""

@@ -61,14 +61,24 @@ void gvv_multi_destroy(void* handle);
  * with the same tuning. */
 int gvv_multi_run(void* handle, const gvv_mesh_io* io, int32_t backward, void* stream);
 
-/* 1: hipGraph replay (default); 0: plain launches of every mesh in order on `stream`. */
+/* 1: hipGraph replay (default); 0: plain launches in order on `stream`. */
 int gvv_multi_use_graph(void* handle, int32_t on);
+
+/* 1: one launch per stage for all meshes -- setup, vertices, depth pass, the shared
+ * large-face pass, resolve, and in the backward setup, the pixel kernel (one launch per tile
+ * edge in use) and the finishing reductions -- each workgroup finding its mesh in a device
+ * table of per-mesh arguments; 0 (default): every mesh's own launches, fanned out over parallel
+ * graph branches.  Results are identical. */
+int gvv_multi_set_batched(void* handle, int32_t on);
 
 /* Autotuner: for every size class present (triangle-count bucket x pixel-count bucket) times
  * forward+backward of the class's largest mesh under each candidate setting (backward tile
  * 8/16/32 x large-face threshold 64/256/1024 px, best of `reps`) and applies the fastest to
  * every mesh of the class.  io must be valid for a backward run.  Overwrites the outputs. */
 int gvv_multi_autotune(void* handle, const gvv_mesh_io* io, int32_t reps, void* stream);
+
+/* Sets the settings of mesh i (as gvv_set_tuning), e.g. from a saved autotuner result. */
+int gvv_multi_set_tuning(void* handle, int32_t mesh, int32_t tile, int32_t large_box);
 
 /* Settings and size class of mesh i. */
 int gvv_multi_tuning(void* handle, int32_t mesh, int32_t* tile, int32_t* large_box, int32_t* size_class);

@@ -44,8 +44,10 @@ def check_against_oracle(mr, grads):
             assert e <= GRAD_TOL, (k, name, e)
 
 
-def test_graph_run_matches_oracle_and_replays(device):
+@pytest.mark.parametrize("batched", [True, False])
+def test_graph_run_matches_oracle_and_replays(device, batched):
     mr = MultiRenderer(small_batch(), device=device)
+    mr.set_batched(batched)
     grads = fill_grads(mr, 1)
     mr.run(backward=True)
     torch.cuda.synchronize()
@@ -64,10 +66,12 @@ def test_graph_run_matches_oracle_and_replays(device):
                 assert rel_err(b[name].cpu().numpy(), f[name].cpu().numpy()) <= 1e-6
 
 
-def test_plain_launches_equal_single_mesh_entry_points(device):
+@pytest.mark.parametrize("batched", [True, False])
+def test_plain_launches_equal_single_mesh_entry_points(device, batched):
     batch = small_batch(seed=3)
     mr = MultiRenderer(batch, device=device)
     mr.use_graph(False)
+    mr.set_batched(batched)
     grads = fill_grads(mr, 2)
     mr.run(backward=True)
     torch.cuda.synchronize()
@@ -96,3 +100,41 @@ def test_autotune_then_graph(device):
     mr.run(backward=True)
     torch.cuda.synchronize()
     check_against_oracle(mr, grads)
+
+
+def test_batched_equals_per_mesh_launches_with_mixed_tiles(device):
+    """The per-stage launches across meshes give the per-mesh launches' results, with meshes on
+    different backward tile edges (one pixel-kernel launch per edge) and large-face thresholds."""
+    batch = small_batch(seed=7)
+    out = []
+    for batched in (False, True):
+        mr = MultiRenderer(batch, device=device)
+        fill_grads(mr, 4)
+        mr.set_batched(batched)
+        for i in range(len(batch)):
+            mr.set_tuning(i, (8, 16, 32)[i % 3], (64, 256, 1024)[(i // 3) % 3])
+        mr.run(backward=True)
+        torch.cuda.synchronize()
+        out.append([{k: v.clone() for k, v in b.items()} for b in mr.bufs])
+    for a, b in zip(*out):
+        for k in ("face", "bary", "render", "vnormal"):
+            assert torch.equal(a[k], b[k]), k
+        for k in ("g_pos", "g_col", "g_sh"):
+            assert rel_err(a[k].cpu().numpy(), b[k].cpu().numpy()) <= 1e-6, k
+
+
+def test_autotune_cache_reused(device, tmp_path):
+    """The autotuner's choices persist in a JSON file keyed by device and mode: a second
+    runtime over meshes of the same size classes applies them without timing."""
+    cache = str(tmp_path / "autotune.json")
+    mr = MultiRenderer(small_batch(seed=5), device=device)
+    fill_grads(mr, 3)
+    assert mr.autotune(reps=1, cache=cache) == "timed"
+    first = [mr.tuning(i) for i in range(len(mr.scenes))]
+    mr2 = MultiRenderer(small_batch(seed=5), device=device)
+    assert mr2.autotune(reps=1, cache=cache) == "cache"
+    assert [mr2.tuning(i) for i in range(len(mr2.scenes))] == first
+    grads = fill_grads(mr2, 6)
+    mr2.run(backward=True)
+    torch.cuda.synchronize()
+    check_against_oracle(mr2, grads)

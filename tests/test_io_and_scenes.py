@@ -85,3 +85,18 @@ def test_config2_batch_shape():
         assert s.faces.min() >= 0 and s.faces.max() < s.N
     Fs = sorted(s.F for s in batch)
     assert Fs[-1] / Fs[0] > 10          # genuinely uneven
+
+
+def test_autotune_cache_file_round_trip(tmp_path):
+    """The multi-mesh autotuner's JSON cache: written atomically, read back, and ignored when
+    its version is unknown."""
+    from gvv_amd import multi
+    path = str(tmp_path / "sub" / "autotune.json")
+    assert multi._read_cache(path) == {}
+    devices = {"gpu|gfx950|256": {"vertexColor/shaded": {"3": {"tile": 8, "large_box": 64}}}}
+    multi._write_cache(path, devices)
+    assert multi._read_cache(path) == devices
+    import json
+    with open(path, "w") as fh:
+        json.dump({"version": 99, "devices": devices}, fh)
+    assert multi._read_cache(path) == {}
