@@ -464,15 +464,6 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
   pixel_grad_body<ALBEDO, SHADED, M2D, HAVE_BARY, TILE>(a, xcd_contiguous(blockIdx.x, gridDim.x));
 }
 
-// The same over several meshes (one tile edge and mode for all of them).  The mesh's arguments
-// are copied from the table first, so they stay in scalar registers across the kernel's stores.
-template <int ALBEDO, bool SHADED, bool M2D, bool HAVE_BARY, int TILE>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_pixel_grad_multi(MeshTable<BackwardArgs> t) {
-  const MeshSlice m = t.find((unsigned)xcd_contiguous(blockIdx.x, gridDim.x));
-  const BackwardArgs a = t.args[m.mesh];
-  pixel_grad_body<ALBEDO, SHADED, M2D, HAVE_BARY, TILE>(a, m.local);
-}
-
 // Fixed-order reduction of the SH tile partials ([b*C + c][27][tiles]): one workgroup per
 // (b, c, coefficient), so the result does not depend on scheduling.
 __device__ __forceinline__ void sh_reduce_block(const float* __restrict__ partial, float* __restrict__ g_sh, int tiles,
@@ -547,7 +538,7 @@ __global__ void __launch_bounds__(256) k_grad_finish(FinishArgs f) { grad_finish
 
 __global__ void __launch_bounds__(256) k_grad_finish_multi(MeshTable<FinishArgs> t) {
   const MeshSlice m = t.find(blockIdx.x);
-  const FinishArgs f = t.args[m.mesh];
+  const FinishArgs f = t.load(m.mesh);
   grad_finish_body(f, (int)m.local);
 }
 
@@ -578,29 +569,7 @@ static void launch_pixel_grad_m(const BackwardArgs& a, hipStream_t s) {
 
 static inline unsigned blocks_for(int64_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
 
-unsigned pixel_grad_blocks(const BackwardArgs& a) { return (unsigned)((int64_t)a.tiles_x * a.tiles_y * a.B * a.C); }
 unsigned finish_blocks(const FinishArgs& f, int C) { return (unsigned)(f.fj_blocks + f.B * C * 27); }
-
-template <int ALBEDO, bool SHADED>
-static void launch_pixel_grad_multi_t(const MeshTable<BackwardArgs>& t, unsigned blocks, int tile, hipStream_t s) {
-  if (tile == 8) hipLaunchKernelGGL((k_pixel_grad_multi<ALBEDO, SHADED, false, false, 8>), dim3(blocks), dim3(64), 0, s, t);
-  else if (tile == 32) hipLaunchKernelGGL((k_pixel_grad_multi<ALBEDO, SHADED, false, false, 32>), dim3(blocks), dim3(64), 0, s, t);
-  else hipLaunchKernelGGL((k_pixel_grad_multi<ALBEDO, SHADED, false, false, 16>), dim3(blocks), dim3(64), 0, s, t);
-}
-
-hipError_t launch_pixel_grad_multi(const MeshTable<BackwardArgs>& t, unsigned blocks, int tile, int albedo, bool shaded,
-                                   hipStream_t s) {
-  if (blocks == 0) return hipSuccess;
-  switch (albedo * 2 + (shaded ? 1 : 0)) {
-    case kVertexColor * 2 + 1: launch_pixel_grad_multi_t<kVertexColor, true>(t, blocks, tile, s); break;
-    case kVertexColor * 2 + 0: launch_pixel_grad_multi_t<kVertexColor, false>(t, blocks, tile, s); break;
-    case kTextured * 2 + 1: launch_pixel_grad_multi_t<kTextured, true>(t, blocks, tile, s); break;
-    case kTextured * 2 + 0: launch_pixel_grad_multi_t<kTextured, false>(t, blocks, tile, s); break;
-    case kForegroundMask * 2 + 1: launch_pixel_grad_multi_t<kForegroundMask, true>(t, blocks, tile, s); break;
-    default: launch_pixel_grad_multi_t<kForegroundMask, false>(t, blocks, tile, s); break;
-  }
-  return hipGetLastError();
-}
 
 hipError_t launch_grad_finish_multi(const MeshTable<FinishArgs>& t, unsigned blocks, hipStream_t s) {
   if (blocks == 0) return hipSuccess;
@@ -621,6 +590,19 @@ SetupArgs backward_setup_args(const BackwardPlan& p) {
   return sa;
 }
 
+hipError_t launch_pixel_grad(const BackwardPlan& p, hipStream_t s) {
+  const BackwardArgs& a = p.a;
+  switch (p.albedo * 2 + (p.shaded ? 1 : 0)) {
+    case kVertexColor * 2 + 1: launch_pixel_grad_m<kVertexColor, true>(a, s); break;
+    case kVertexColor * 2 + 0: launch_pixel_grad_m<kVertexColor, false>(a, s); break;
+    case kTextured * 2 + 1: launch_pixel_grad_m<kTextured, true>(a, s); break;
+    case kTextured * 2 + 0: launch_pixel_grad_m<kTextured, false>(a, s); break;
+    case kForegroundMask * 2 + 1: launch_pixel_grad_m<kForegroundMask, true>(a, s); break;
+    default: launch_pixel_grad_m<kForegroundMask, false>(a, s); break;
+  }
+  return hipGetLastError();
+}
+
 hipError_t launch_backward(const BackwardPlan& p, hipStream_t s) {
   const BackwardArgs& a = p.a;
   {
@@ -631,14 +613,8 @@ hipError_t launch_backward(const BackwardPlan& p, hipStream_t s) {
   }
   {
     ProfScope ps(p.prof, "k_pixel_grad", s);
-    switch (p.albedo * 2 + (p.shaded ? 1 : 0)) {
-      case kVertexColor * 2 + 1: launch_pixel_grad_m<kVertexColor, true>(a, s); break;
-      case kVertexColor * 2 + 0: launch_pixel_grad_m<kVertexColor, false>(a, s); break;
-      case kTextured * 2 + 1: launch_pixel_grad_m<kTextured, true>(a, s); break;
-      case kTextured * 2 + 0: launch_pixel_grad_m<kTextured, false>(a, s); break;
-      case kForegroundMask * 2 + 1: launch_pixel_grad_m<kForegroundMask, true>(a, s); break;
-      default: launch_pixel_grad_m<kForegroundMask, false>(a, s); break;
-    }
+    hipError_t e = launch_pixel_grad(p, s);
+    if (e != hipSuccess) return e;
   }
   if (p.shaded) {
     ProfScope ps(p.prof, "k_grad_finish", s);

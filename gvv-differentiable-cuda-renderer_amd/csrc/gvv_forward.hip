@@ -72,7 +72,7 @@ __global__ void __launch_bounds__(256) k_setup(SetupGrid g) { setup_body(g, bloc
 
 __global__ void __launch_bounds__(256) k_setup_multi(MeshTable<SetupGrid> t) {
   const MeshSlice m = t.find(blockIdx.x);
-  const SetupGrid g = t.args[m.mesh];
+  const SetupGrid g = t.load(m.mesh);
   setup_body(g, (int)m.local);
 }
 
@@ -169,7 +169,7 @@ __global__ void __launch_bounds__(256) k_vertices(VertexArgs va) { vertices_body
 // scalar registers across the kernel's stores.
 __global__ void __launch_bounds__(256) k_vertices_multi(MeshTable<VertexArgs> t) {
   const MeshSlice m = t.find(blockIdx.x);
-  const VertexArgs a = t.args[m.mesh];
+  const VertexArgs a = t.load(m.mesh);
   vertices_body(a, m.local);
 }
 
@@ -395,7 +395,7 @@ __global__ void __launch_bounds__(64) k_raster(RasterArgs ra) { raster_body(ra, 
 
 __global__ void __launch_bounds__(64) k_raster_multi(MeshTable<RasterArgs> t) {
   const MeshSlice m = t.find((unsigned)xcd_contiguous(blockIdx.x, gridDim.x));
-  const RasterArgs a = t.args[m.mesh];
+  const RasterArgs a = t.load(m.mesh);
   raster_body(a, m.local);
 }
 
@@ -542,7 +542,7 @@ __global__ void __launch_bounds__(256) k_resolve(ResolveArgs a) {
 template <int ALBEDO, bool SHADE>
 __global__ void __launch_bounds__(256) k_resolve_multi(MeshTable<ResolveArgs> t) {
   const MeshSlice m = t.find((unsigned)xcd_contiguous(blockIdx.x, gridDim.x));
-  const ResolveArgs a = t.args[m.mesh];
+  const ResolveArgs a = t.load(m.mesh);
   resolve_body<ALBEDO, SHADE>(a, (unsigned)m.local);
 }
 

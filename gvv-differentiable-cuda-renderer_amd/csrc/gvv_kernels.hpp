@@ -74,6 +74,11 @@ __device__ __forceinline__ int64_t xcd_contiguous(int64_t wg, int64_t n) {
 // One launch over several meshes: mesh i owns the blocks [start[i], start[i+1]) of the grid
 // (in the logical order, after any XCD remapping) and takes args[i].  find() is a binary search
 // over the uniform prefix table.
+// The table is read through the constant address space, so that its workgroup-uniform values
+// are scalar loads into scalar registers: through a generic pointer the compiler cannot prove
+// the table unchanged by the kernels' own stores, and keeps the loaded arguments in vector
+// registers, where they crowd out the kernel's working set.
+typedef const __attribute__((address_space(4))) int* ConstWords;
 struct MeshSlice { int mesh; int64_t local; };
 template <class Args>
 struct MeshTable {
@@ -82,15 +87,26 @@ struct MeshTable {
   int n;
   __device__ __forceinline__ MeshSlice find(unsigned wg) const {
     int lo = 0, hi = n - 1;
+    const ConstWords st = (ConstWords)start;
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
-      if (start[mid] <= wg) lo = mid;
+      if ((unsigned)st[mid] <= wg) lo = mid;
       else hi = mid - 1;
     }
     MeshSlice r;
     r.mesh = lo;
-    r.local = (int64_t)(wg - start[lo]);
+    r.local = (int64_t)(wg - (unsigned)st[lo]);
     return r;
+  }
+  // the mesh's arguments, by scalar loads
+  __device__ __forceinline__ Args load(int mesh) const {
+    static_assert(sizeof(Args) % 4 == 0, "argument records are whole 32-bit words");
+    Args a;
+    const ConstWords src = (ConstWords)(args + mesh);
+    int* dst = reinterpret_cast<int*>(&a);
+#pragma unroll
+    for (int k = 0; k < (int)(sizeof(Args) / 4); ++k) dst[k] = src[k];
+    return a;
   }
 };
 
@@ -286,20 +302,19 @@ hipError_t launch_clear_counters(unsigned int* counters, hipStream_t s);
 hipError_t launch_raster_large_multi(const RasterMesh* meshes, unsigned int* counters, const int4* large,
                                      hipStream_t s);
 hipError_t launch_backward(const BackwardPlan& p, hipStream_t s);
+// The backward's pixel kernel alone (no setup, no finishing reductions).
+hipError_t launch_pixel_grad(const BackwardPlan& p, hipStream_t s);
 
 // Stage launches over several meshes (the multi-mesh runtime): one launch per stage, each
 // workgroup finding its mesh in the table.  `blocks` is the table's total block count.
 SetupArgs backward_setup_args(const BackwardPlan& p);
 unsigned resolve_blocks(const ResolveArgs& a);
-unsigned pixel_grad_blocks(const BackwardArgs& a);
 unsigned finish_blocks(const FinishArgs& f, int C);
 hipError_t launch_setup_multi(const MeshTable<SetupGrid>& t, unsigned blocks, hipStream_t s);
 hipError_t launch_vertices_multi(const MeshTable<VertexArgs>& t, unsigned blocks, hipStream_t s);
 hipError_t launch_raster_multi(const MeshTable<RasterArgs>& t, unsigned blocks, hipStream_t s);
 hipError_t launch_resolve_multi(const MeshTable<ResolveArgs>& t, unsigned blocks, int albedo, bool shade,
                                 hipStream_t s);
-hipError_t launch_pixel_grad_multi(const MeshTable<BackwardArgs>& t, unsigned blocks, int tile, int albedo, bool shaded,
-                                   hipStream_t s);
 hipError_t launch_grad_finish_multi(const MeshTable<FinishArgs>& t, unsigned blocks, hipStream_t s);
 // The tile-binned depth pass (gvv_tile.hip): binning (after k_vertices), and the fused
 // per-tile raster + resolve (after the large-face pass).

@@ -54,8 +54,7 @@ int size_class(const Handle* h) {
 // The per-stage argument tables of a batched run (one launch per stage across all meshes):
 // for each stage the meshes' argument records and the prefix of their block counts, packed
 // into one device buffer (uploaded when it changes).
-enum Stage { kFwdSetup, kVertices, kRaster, kResolve, kBwdSetup, kPixelGrad8, kPixelGrad16, kPixelGrad32, kFinish,
-             kStages };
+enum Stage { kFwdSetup, kVertices, kRaster, kResolve, kBwdSetup, kFinish, kStages };
 struct StageTables {
   std::vector<unsigned char> host;
   DevBuf<unsigned char> dev;
@@ -201,23 +200,17 @@ int build_stages(Multi* m, const std::vector<ForwardPlan>& fp, const std::vector
   if (backward) {
     std::vector<SetupGrid> bsetup(n);
     std::vector<unsigned> b_bsetup(n);
-    std::vector<BackwardArgs> grad[3];
-    std::vector<unsigned> b_grad[3];
     std::vector<FinishArgs> fin;
     std::vector<unsigned> b_fin;
     for (int i = 0; i < n; ++i) {
       bsetup[i] = setup_grid(backward_setup_args(bp[i]));
       b_bsetup[i] = (unsigned)(bsetup[i].cam_blocks + bsetup[i].face_blocks + bsetup[i].fill_blocks);
-      const int g = bp[i].a.tile == 8 ? 0 : bp[i].a.tile == 16 ? 1 : 2;
-      grad[g].push_back(bp[i].a);
-      b_grad[g].push_back(pixel_grad_blocks(bp[i].a));
       if (bp[i].shaded) {
         fin.push_back(finish_args(bp[i].a));
         b_fin.push_back(finish_blocks(fin.back(), bp[i].a.C));
       }
     }
     put_stage(st, blob, kBwdSetup, bsetup, b_bsetup);
-    for (int g = 0; g < 3; ++g) put_stage(st, blob, kPixelGrad8 + g, grad[g], b_grad[g]);
     put_stage(st, blob, kFinish, fin, b_fin);
   }
   if (blob != st.host || !st.dev.p) {
@@ -231,9 +224,12 @@ int build_stages(Multi* m, const std::vector<ForwardPlan>& fp, const std::vector
 }
 
 // One launch per stage across all meshes: clear the queue, setup, vertices, depth pass, the
-// shared large-face launch, resolve; then the backward's setup, pixel kernels (one launch per
-// tile edge in use) and finishing reductions.
-hipError_t enqueue_batched(Multi* m, bool backward, hipStream_t s) {
+// shared large-face launch, resolve; then the backward's setup, pixel kernels and finishing
+// reductions.  The pixel kernels are the one stage still launched per mesh (through `pixel`,
+// which may fan them out over the graph's branches): across meshes they would keep the mesh's
+// arguments in registers the single-mesh kernel reloads from its argument segment, and spill.
+template <class PixelStage>
+hipError_t enqueue_batched(Multi* m, bool backward, hipStream_t s, PixelStage&& pixel) {
   const StageTables& st = m->stages;
   const bool shade = (m->shading == kShaded) && m->albedo != kNormal;
   hipError_t e = launch_clear_counters(m->counters.p, s);
@@ -244,9 +240,7 @@ hipError_t enqueue_batched(Multi* m, bool backward, hipStream_t s) {
   if (e == hipSuccess) e = launch_resolve_multi(st.table<ResolveArgs>(kResolve), st.blocks[kResolve], m->albedo, shade, s);
   if (!backward) return e;
   if (e == hipSuccess) e = launch_setup_multi(st.table<SetupGrid>(kBwdSetup), st.blocks[kBwdSetup], s);
-  for (int g = 0; g < 3 && e == hipSuccess; ++g)
-    e = launch_pixel_grad_multi(st.table<BackwardArgs>(kPixelGrad8 + g), st.blocks[kPixelGrad8 + g], 8 << g, m->albedo,
-                                m->shading == kShaded, s);
+  if (e == hipSuccess) e = pixel();
   if (e == hipSuccess) e = launch_grad_finish_multi(st.table<FinishArgs>(kFinish), st.blocks[kFinish], s);
   return e;
 }
@@ -355,7 +349,9 @@ int capture(Multi* m, const gvv_mesh_io* io, bool backward, const std::vector<Fo
   MULTI_CHECK(hipStreamBeginCapture(m->lanes[0], hipStreamCaptureModeThreadLocal));
   hipError_t e = hipSuccess;
   if (m->batched) {
-    e = enqueue_batched(m, backward, m->lanes[0]);
+    e = enqueue_batched(m, backward, m->lanes[0], [&]() {
+      return fan_out(m, order, io, 1, [&](int i, hipStream_t s) { return launch_pixel_grad(bp[i], s); });
+    });
   } else {
     e = launch_clear_counters(m->counters.p, m->lanes[0]);
     if (e == hipSuccess)
@@ -447,8 +443,15 @@ int gvv_multi_run(void* handle, const gvv_mesh_io* io, int32_t backward, void* s
   std::vector<BackwardPlan> bp;
   if (plan_all(m, io, backward != 0, s, fp, bp)) return 1;
   if (!m->use_graph) {
-    if (m->batched) MULTI_CHECK(enqueue_batched(m, backward != 0, s));
-    else MULTI_CHECK(enqueue_all(m, fp, bp, backward != 0, s));
+    if (m->batched) {
+      MULTI_CHECK(enqueue_batched(m, backward != 0, s, [&]() {
+        hipError_t e = hipSuccess;
+        for (size_t i = 0; i < bp.size() && e == hipSuccess; ++i) e = launch_pixel_grad(bp[i], s);
+        return e;
+      }));
+    } else {
+      MULTI_CHECK(enqueue_all(m, fp, bp, backward != 0, s));
+    }
     return 0;
   }
   if (!m->exec || m->key != graph_key(m, io, backward != 0))

@@ -2,9 +2,10 @@
 
 Architecture (all convolutions 3x3, stride 2, pad 1, ReLU; NHWC bf16 activations):
 
-    input  256 x 256 x 8   mosaic of the 4 views, each box-filtered from the fit resolution to
-                           128 x 128: RGB - 0.5, covered fraction, tile-local x and y, 2 zero
-                           channels (features=1; features=0 is plain RGB + 5 zero channels)
+    input  256 x 256 RGB   the 4 views as one 256 x 256 render: each box-filtered from the fit
+                           resolution to 128 x 128 and tiled 2 x 2, RGB - 0.5, stored as 8 bf16
+                           channels of which 5 are zero (features=0, the default); features=1
+                           adds the covered fraction and tile-local x and y
     conv1  128 x 128 x 32
     conv2   64 x  64 x 64
     conv3   32 x  32 x 128
@@ -39,7 +40,7 @@ N_OUT_PAD = 128
 LAYERS = ((8, 32), (32, 64), (64, 128), (128, 128), (128, 256), (256, 256))   # (Cin, Cout), 3x3 s2 p1
 INPUT_RES = 256
 VIEW_RES = 128
-FEATURES = 1                      # input channels: 1 = RGB - 0.5, coverage, tile x/y; 0 = RGB only
+FEATURES = 0                      # input channels: 0 = RGB only; 1 = RGB - 0.5, coverage, tile x/y
 
 
 def _ptr(t):

@@ -65,10 +65,10 @@ int gvv_multi_run(void* handle, const gvv_mesh_io* io, int32_t backward, void* s
 int gvv_multi_use_graph(void* handle, int32_t on);
 
 /* 1: one launch per stage for all meshes -- setup, vertices, depth pass, the shared
- * large-face pass, resolve, and in the backward setup, the pixel kernel (one launch per tile
- * edge in use) and the finishing reductions -- each workgroup finding its mesh in a device
- * table of per-mesh arguments; 0 (default): every mesh's own launches, fanned out over parallel
- * graph branches.  Results are identical. */
+ * large-face pass, resolve, and in the backward setup and the finishing reductions -- each
+ * workgroup finding its mesh in a device table of per-mesh arguments; the backward's pixel
+ * kernels stay per mesh, fanned out over parallel graph branches.  0 (default): every mesh's own
+ * launches for every stage, fanned out over the branches.  Results are identical. */
 int gvv_multi_set_batched(void* handle, int32_t on);
 
 /* Autotuner: for every size class present (triangle-count bucket x pixel-count bucket) times
