@@ -5,37 +5,64 @@
 //
 //   k_pixel_grad   one wavefront per TILE x TILE pixel tile (TILE 8/16/32, in 8x8 rounds).
 //                  Each covered pixel forms its albedo, SH, barycentric-position,
-//                  model-to-data and normal terms.  Per-face terms are summed in LDS slots
-//                  (one per distinct face of the tile) before lane-parallel global atomics
-//                  flush them to the face's three vertices; SH terms accumulate in registers
-//                  over the tile and are reduced once per tile (DPP) into a partial buffer.
-//   k_grad_finish  one launch of two parts.  SH: fixed-order reduction of the SH partials
-//                  (deterministic).  Face Jacobian: the normal -> position chain.  Instead of walking every incident face of
-//                  every corner for every pixel (Grad.cu:559-615), the pixels accumulate
-//                  dL/d(vertex normal) per vertex; one thread per face then forms
-//                  G = sum of its vertices' normal gradients and scatters G x (vj - vi),
-//                  (vk - vi) x G and -(both) -- the same sum, regrouped by face.
+//                  model-to-data and normal terms, from the face's ray/triangle and Jacobian
+//                  constants (FaceGrad, one 128-byte line per face, written by the setup
+//                  launch).  Per-face terms are summed in LDS slots (one per distinct face of
+//                  the tile) before lane-parallel global atomics flush them to the face's three
+//                  vertices; in the shaded modes a vertex's position and normal terms share one
+//                  24-byte row of a scratch buffer, so that they cost one atomic request rather
+//                  than two.  SH terms accumulate in registers over the tile and are reduced
+//                  once per tile (a transposed DPP sum) into a partial buffer.
+//   k_grad_finish  one launch of three parts.  Face Jacobian: the normal -> position chain.
+//                  Instead of walking every incident face of every corner for every pixel
+//                  (Grad.cu:559-615), the pixels accumulate dL/d(vertex normal) per vertex; one
+//                  thread per face then forms G = sum of its vertices' normal gradients and
+//                  scatters G x (vj - vi), (vk - vi) x G and -(both) -- the same sum, regrouped
+//                  by face.  Position terms: the pixels' position terms from the scratch rows
+//                  into the position gradient.  SH: fixed-order reduction of the SH partials
+//                  (deterministic).
 #include "gvv_bary_grad.hpp"
 #include "gvv_kernels.hpp"
 
 namespace gvv {
 
-// Sum of v over the 64 lanes, returned in every lane.  Inclusive DPP scan: row_shr 1/2/4/8
-// within each row of 16, then row_bcast 15 and 31 across rows; lane 63 ends with the total,
-// which readlane broadcasts.  All DPP steps are full-rate VALU adds.
 template <int CTRL, int ROW_MASK, bool BOUND_ZERO>
 __device__ __forceinline__ float dpp_mov(float v) {
   return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xf, BOUND_ZERO));
 }
 
-__device__ __forceinline__ float wave_total(float v) {
-  v += dpp_mov<0x111, 0xf, true>(v);    // row_shr:1
-  v += dpp_mov<0x112, 0xf, true>(v);    // row_shr:2
-  v += dpp_mov<0x114, 0xf, true>(v);    // row_shr:4
-  v += dpp_mov<0x118, 0xf, true>(v);    // row_shr:8
-  v += dpp_mov<0x142, 0xa, false>(v);   // row_bcast:15 into rows 1 and 3
-  v += dpp_mov<0x143, 0xc, false>(v);   // row_bcast:31 into rows 2 and 3
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+// One halving step of a transposed wave sum: of the 2H values r[0..2H), a lane keeps the half
+// that its lane bit MASK selects (upper half when set) and adds the partner's (lane ^ MASK)
+// copy of that half, which the partner gives away.  XOR_FROM reads lane ^ MASK (same-row
+// permutations: a DPP control; across rows: a shuffle).
+template <int H, class XorFrom>
+__device__ __forceinline__ void halve(float* r, bool upper, XorFrom xor_from) {
+#pragma unroll
+  for (int j = 0; j < H; ++j) {
+    const float keep = upper ? r[H + j] : r[j];
+    const float give = upper ? r[j] : r[H + j];
+    r[j] = keep + xor_from(give);
+  }
+}
+
+// The sums over the 64 lanes of the 27 values v[0..27), transposed: lane L returns the total of
+// value L >> 1 (for L < 54; lanes L and L ^ 1 hold the same total).  Five halving exchanges
+// (lane bits 5 to 1) leave each lane one value summed over 32 lanes; one more adds the pair.
+// All 64 lanes must be active.  The summation order is fixed, so the result is deterministic.
+__device__ __forceinline__ float wave_sums27(const float* v, int lane) {
+  float r[32];
+#pragma unroll
+  for (int k = 0; k < 32; ++k) r[k] = k < 27 ? v[k] : 0.f;
+  halve<16>(r, lane & 32, [](float x) { return __shfl_xor(x, 32); });
+  halve<8>(r, lane & 16, [](float x) { return __shfl_xor(x, 16); });
+  halve<4>(r, lane & 8, [](float x) { return dpp_mov<0x128, 0xf, false>(x); });       // row_ror:8
+  const bool low4 = (lane & 4) == 0;
+  halve<2>(r, !low4, [low4](float x) {
+    const float up = dpp_mov<0x104, 0xf, false>(x), down = dpp_mov<0x114, 0xf, false>(x);   // row_shl:4 / row_shr:4
+    return low4 ? up : down;
+  });
+  halve<1>(r, lane & 2, [](float x) { return dpp_mov<0x4E, 0xf, false>(x); });        // quad_perm [2,3,0,1]
+  return r[0] + dpp_mov<0xB1, 0xf, false>(r[0]);                                     // quad_perm [1,0,3,2]
 }
 
 // Merges the per-face terms of lane pairs (lane, lane ^ MASK) that add to the same face slot:
@@ -61,9 +88,10 @@ __device__ __forceinline__ void merge_pairs(float* t, int& slot, int lane) {
 // three barycentrics), added to out[OFF .. OFF+8] (gvv_bary_grad.hpp; reference dJBCDVerpos,
 // RendererUtil.h:670-861).  Binary32 throughout: the ill-conditioned plane hit is formed with
 // error-free transformations.
+// `f` holds the face's constants (bary_face_setup of the same vertices), v0 its first vertex.
 template <int OFF>
-__device__ __forceinline__ void bary_position_grad(V3 o, V3 d, V3 v0, V3 v1, V3 v2, V3 gb, float* out) {
-  const BaryGradSetup g = bary_grad_setup(o, d, v0, v1, v2);
+__device__ __forceinline__ void bary_position_grad(V3 o, V3 d, V3 v0, const BaryFace& f, V3 gb, float* out) {
+  const BaryGradSetup g = bary_grad_pixel(f, o, d, v0);
   if (!g.live) return;
   bary_grad_apply(g, d, gb, __builtin_amdgcn_rcpf(g.den), __builtin_amdgcn_rcpf(g.nd), out + OFF);
 }
@@ -104,7 +132,15 @@ __device__ __forceinline__ void pixel_terms(const BackwardArgs& a, const Camera&
   // Every load that does not depend on the barycentrics is issued before they are computed,
   // so that all of them share one round trip to memory; only the texels wait for a second.
   const float* vp = a.vpos + (size_t)b * a.N * 3;
-  const V3 p0 = ld3(vp + 3 * i0), p1 = ld3(vp + 3 * i1), p2 = ld3(vp + 3 * i2);
+  const V3 p0 = ld3(vp + 3 * i0);
+  V3 p1, p2;
+  if (M2D) { p1 = ld3(vp + 3 * i1); p2 = ld3(vp + 3 * i2); }
+  // the face's ray/triangle and Jacobian constants (one 128-byte line, written by the setup)
+  const FaceGrad* fr = a.face_grad + (size_t)b * a.F + idf;
+  TriSetup ts;
+  if (!HAVE_BARY) ts = fr->t;
+  BaryFace bf;
+  if (SHADED) bf = fr->g;
   const float* vnb = a.vnormal + (size_t)bc * a.N * 3;
   const V3 n0 = ld3(vnb + 3 * i0), n1 = ld3(vnb + 3 * i1), n2 = ld3(vnb + 3 * i2);
   const float* gr = a.g_render + 3 * idx;
@@ -126,7 +162,7 @@ __device__ __forceinline__ void pixel_terms(const BackwardArgs& a, const Camera&
   const V3 d = pixel_ray(cam, px + 0.5f, py + 0.5f);
   V3 abc;
   if (HAVE_BARY) abc = v3(ba, bb, 1.f - ba - bb);
-  else abc = ray_barycentric(tri_setup(p0, p1, p2), cam_origin_scaled(cam), d);
+  else abc = ray_barycentric(ts, cam_origin_scaled(cam), d);
   const float w3[3] = {abc.x, abc.y, abc.z};
   // textured: the four texels of the backward's blend (Grad.cu:282-312; not the forward's
   // nearest texel) are loaded before the normal and the illumination are formed, which do not
@@ -199,7 +235,7 @@ __device__ __forceinline__ void pixel_terms(const BackwardArgs& a, const Camera&
 #pragma unroll
     for (int r = 0; r < 3; ++r) { t[G_VN + 3 * r] = gu.x * w3[r]; t[G_VN + 3 * r + 1] = gu.y * w3[r]; t[G_VN + 3 * r + 2] = gu.z * w3[r]; }
     const V3 gb = v3(dot(f3(gu), f3(n0)), dot(f3(gu), f3(n1)), dot(f3(gu), f3(n2)));
-    bary_position_grad<G_POS>(o, d, p0, p1, p2, gb, t);
+    bary_position_grad<G_POS>(o, d, p0, bf, gb, t);
   }
 
   if (M2D) {
@@ -284,8 +320,10 @@ __device__ __forceinline__ void pixel_grad_body(const BackwardArgs& a, int64_t w
   const int lane = threadIdx.x;
   const int tiles = a.tiles_x * a.tiles_y;
   // one (view, tile) per workgroup
-  const int tile = __builtin_amdgcn_readfirstlane((int)(work % tiles));
-  const int bc = __builtin_amdgcn_readfirstlane((int)(work / tiles));
+  // the grid has fewer than 2^32 workgroups: 32-bit quotient and remainder
+  const unsigned w32 = (unsigned)work;
+  const int tile = __builtin_amdgcn_readfirstlane((int)(w32 % (unsigned)tiles));
+  const int bc = __builtin_amdgcn_readfirstlane((int)(w32 / (unsigned)tiles));
   const int b = __builtin_amdgcn_readfirstlane(bc / a.C);
   const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
   // the view's camera and SH coefficients: uniform, loaded once before any store of the kernel
@@ -325,8 +363,11 @@ __device__ __forceinline__ void pixel_grad_body(const BackwardArgs& a, int64_t w
       if (v == 0.f) continue;
       const int grp = k / 9, kk = k % 9;
       const int vtx = a.faces[3 * s_face[slot] + kk / 3];
-      float* base = grp * 9 == G_POS && NEED_POS ? a.g_pos : (grp * 9 == G_VN && SHADED ? a.g_vn : a.g_col);
-      atomicAdd(base + ((size_t)b * a.N + vtx) * 3 + kk % 3, v);
+      const size_t row = (size_t)b * a.N + vtx;
+      float* dst;
+      if (SHADED && grp < 2) dst = a.g_pv + row * 6 + 3 * grp + kk % 3;   // position, normal: one 24-byte row
+      else dst = (grp * 9 == G_POS && NEED_POS ? a.g_pos : a.g_col) + row * 3 + kk % 3;
+      atomicAdd(dst, v);
     }
     __syncthreads();
     nslots = 0;
@@ -451,11 +492,10 @@ __device__ __forceinline__ void pixel_grad_body(const BackwardArgs& a, int64_t w
   if (NT > 0 && nslots > 0) flush();
 
   if (SHADED) {
-#pragma unroll
-    for (int k = 0; k < 27; ++k) {
-      const float s = any ? wave_total(sh_acc[k]) : 0.f;
-      if (lane == k) a.sh_partial[((size_t)bc * 27 + k) * tiles + tile] = s;
-    }
+    // lane 2k stores the tile's total of coefficient k
+    const float s = any ? wave_sums27(sh_acc, lane) : 0.f;
+    const int k = lane >> 1;
+    if ((lane & 1) == 0 && k < 27) a.sh_partial[((size_t)bc * 27 + k) * tiles + tile] = s;
   }
 }
 
@@ -484,7 +524,7 @@ __device__ __forceinline__ void sh_reduce_block(const float* __restrict__ partia
 // Normal -> position chain regrouped by face (see the file header), for the workgroup's 256
 // faces.
 __device__ __forceinline__ void face_jac_block(const float* __restrict__ vpos, const int32_t* __restrict__ faces,
-                                               const float* __restrict__ g_vn, float* __restrict__ g_pos, int B, int N,
+                                               const float* __restrict__ g_pv, float* __restrict__ g_pos, int B, int N,
                                                int F, int blk, float* smem) {
   // the workgroup's 256 faces x 3 corners x 3 channels are staged in LDS and added with lane q
   // taking channel q mod 3 of corner q / 3, so that a vertex's three channels are adjacent
@@ -500,10 +540,10 @@ __device__ __forceinline__ void face_jac_block(const float* __restrict__ vpos, c
     s_b[t] = b;
     const int f = (int)(i % F);
     const int a0 = faces[3 * f], a1 = faces[3 * f + 1], a2 = faces[3 * f + 2];
-    const float* gv = g_vn + (size_t)b * N * 3;
-    V3 G = ld3(gv + 3 * a0);
-    if (a1 != a0) G = G + ld3(gv + 3 * a1);
-    if (a2 != a0 && a2 != a1) G = G + ld3(gv + 3 * a2);
+    const float* gv = g_pv + (size_t)b * N * 6 + 3;     // the normal-gradient half of each row
+    V3 G = ld3(gv + 6 * a0);
+    if (a1 != a0) G = G + ld3(gv + 6 * a1);
+    if (a2 != a0 && a2 != a1) G = G + ld3(gv + 6 * a2);
     if (!(G.x == 0.f && G.y == 0.f && G.z == 0.f)) {
       const float* vp = vpos + (size_t)b * N * 3;
       const V3 vi = ld3(vp + 3 * a0), vj = ld3(vp + 3 * a1), vk = ld3(vp + 3 * a2);
@@ -526,12 +566,33 @@ __device__ __forceinline__ void face_jac_block(const float* __restrict__ vpos, c
   }
 }
 
-// The two reductions after the pixel kernel in one launch: blocks [0, fj_blocks) run the
-// face Jacobian, the rest the SH reduction (one per view and coefficient).
+// The pixels' position terms of the workgroup's 768 position-gradient floats, added from the
+// interleaved rows (3 per thread, consecutive floats on consecutive lanes).
+__device__ __forceinline__ void pos_terms_block(const float* __restrict__ g_pv, float* __restrict__ g_pos,
+                                                int64_t n, int blk) {
+  for (int q = threadIdx.x; q < 768; q += 256) {
+    const int64_t i = (int64_t)blk * 768 + q;
+    if (i >= n) return;
+    const float v = g_pv[(i / 3) * 6 + i % 3];
+    if (v != 0.f) atomicAdd(g_pos + i, v);
+  }
+}
+
+// The reductions after the pixel kernel in one launch: blocks [0, fj_blocks) run the face
+// Jacobian, the next pv_blocks add the pixels' position terms, the rest run the SH reduction
+// (one per view and coefficient).  The first two add into the zero-filled position gradient.
 __device__ __forceinline__ void grad_finish_body(const FinishArgs& f, int blk) {
   __shared__ float smem[256 * 13];
-  if (blk < f.fj_blocks) face_jac_block(f.vpos, f.faces, f.g_vn, f.g_pos, f.B, f.N, f.F, blk, smem);
-  else sh_reduce_block(f.sh_partial, f.g_sh, f.tiles, blk - f.fj_blocks, smem);
+  if (blk < f.fj_blocks) {
+    face_jac_block(f.vpos, f.faces, f.g_pv, f.g_pos, f.B, f.N, f.F, blk, smem);
+    return;
+  }
+  blk -= f.fj_blocks;
+  if (blk < f.pv_blocks) {
+    pos_terms_block(f.g_pv, f.g_pos, (int64_t)f.B * f.N * 3, blk);
+    return;
+  }
+  sh_reduce_block(f.sh_partial, f.g_sh, f.tiles, blk - f.pv_blocks, smem);
 }
 
 __global__ void __launch_bounds__(256) k_grad_finish(FinishArgs f) { grad_finish_body(f, blockIdx.x); }
@@ -544,9 +605,10 @@ __global__ void __launch_bounds__(256) k_grad_finish_multi(MeshTable<FinishArgs>
 
 FinishArgs finish_args(const BackwardArgs& a) {
   FinishArgs f;
-  f.vpos = a.vpos; f.faces = a.faces; f.g_vn = a.g_vn; f.g_pos = a.g_pos; f.sh_partial = a.sh_partial; f.g_sh = a.g_sh;
+  f.vpos = a.vpos; f.faces = a.faces; f.g_pv = a.g_pv; f.g_pos = a.g_pos; f.sh_partial = a.sh_partial; f.g_sh = a.g_sh;
   f.B = a.B; f.N = a.N; f.F = a.F; f.tiles = a.tiles_x * a.tiles_y;
   f.fj_blocks = (int)(((int64_t)a.B * a.F + 255) / 256);
+  f.pv_blocks = (int)(((int64_t)a.B * a.N * 3 + 767) / 768);
   return f;
 }
 
@@ -569,7 +631,7 @@ static void launch_pixel_grad_m(const BackwardArgs& a, hipStream_t s) {
 
 static inline unsigned blocks_for(int64_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
 
-unsigned finish_blocks(const FinishArgs& f, int C) { return (unsigned)(f.fj_blocks + f.B * C * 27); }
+unsigned finish_blocks(const FinishArgs& f, int C) { return (unsigned)(f.fj_blocks + f.pv_blocks + f.B * C * 27); }
 
 hipError_t launch_grad_finish_multi(const MeshTable<FinishArgs>& t, unsigned blocks, hipStream_t s) {
   if (blocks == 0) return hipSuccess;
@@ -582,10 +644,13 @@ SetupArgs backward_setup_args(const BackwardPlan& p) {
   const int64_t nv = (int64_t)a.B * a.N * 3;
   SetupArgs sa;
   sa.extr = p.extr; sa.intr = p.intr; sa.cams = p.cams; sa.n_cams = a.B * a.C;
+  // the per-face constants of the pixel kernel, once per face and batch entry
+  sa.vpos = a.vpos; sa.faces = a.faces; sa.face_grad = const_cast<FaceGrad*>(a.face_grad);
+  sa.B = a.B; sa.N = a.N; sa.F = a.F;
   sa.fill[0] = a.g_pos; sa.fill_n[0] = nv;
   sa.fill[1] = a.g_col; sa.fill_n[1] = nv;
   sa.fill[2] = a.g_tex; sa.fill_n[2] = (int64_t)a.B * a.texH * a.texW * 3;
-  if (p.shaded) { sa.fill[3] = a.g_vn; sa.fill_n[3] = nv; }
+  if (p.shaded) { sa.fill[3] = a.g_pv; sa.fill_n[3] = 2 * nv; }
   else { sa.fill[3] = a.g_sh; sa.fill_n[3] = (int64_t)a.B * a.C * 27; }
   return sa;
 }
@@ -619,7 +684,7 @@ hipError_t launch_backward(const BackwardPlan& p, hipStream_t s) {
   if (p.shaded) {
     ProfScope ps(p.prof, "k_grad_finish", s);
     const FinishArgs f = finish_args(a);
-    hipLaunchKernelGGL(k_grad_finish, dim3(f.fj_blocks + a.B * a.C * 27), dim3(256), 0, s, f);
+    hipLaunchKernelGGL(k_grad_finish, dim3(finish_blocks(f, a.C)), dim3(256), 0, s, f);
   }
   return hipGetLastError();
 }

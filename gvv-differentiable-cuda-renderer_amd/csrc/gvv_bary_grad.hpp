@@ -88,31 +88,51 @@ GVV_HD void dot2(V3 a, V3 b, float& hi, float& lo) {
   two_sum(p, s, hi, lo);
 }
 
-// The plane hit and the triangle constants of the Jacobian for a ray from o along d.
-GVV_HD BaryGradSetup bary_grad_setup(V3 o, V3 d, V3 v0, V3 v1, V3 v2) {
+// The per-face constants of the Jacobian, shared by every pixel and view of the face (64 bytes:
+// four 16-byte loads).  den < 0 marks a face the reference's early exit always takes.
+struct alignas(16) BaryFace {
+  V3 e1, e2;     // edges v1 - v0, v2 - v0 (binary32)
+  V3 Nh, Nl;     // e1 x e2 to twice the working precision, from the exact edges
+  float den;     // Nh . Nh, or -1 for a degenerate face
+  V3 nN;         // unit binary32 normal normalize(e1 x e2), for the early exit's angle test
+};
+static_assert(sizeof(BaryFace) == 64, "four 16-byte loads");
+
+GVV_HD BaryFace bary_face_setup(V3 v0, V3 v1, V3 v2) {
+  BaryFace f;
+  // the reference's early-exit quantities in binary32 (decisions)
+  const V3 e1 = v1 - v0, e2 = v2 - v0;
+  const V3 N = cross(e1, e2);
+  const float denf = dot(N, N);
+  const bool degenerate = fabsf(denf * denf) < 0.001f;
+  f.nN = degenerate ? v3(0.f, 0.f, 0.f) : normalize(N);
+  // the edges exactly (e = eh + el; eh is the binary32 edge) and their cross product to twice
+  // the working precision: the plane's tilt enters the hit as |q0| |dN| / (|N| cos), so N must
+  // be far more accurate than binary32 for the hit to be accurate to binary32
+  V3 e1l, e2l;
+  two_sum(v1.x, -v0.x, f.e1.x, e1l.x);
+  two_sum(v1.y, -v0.y, f.e1.y, e1l.y);
+  two_sum(v1.z, -v0.z, f.e1.z, e1l.z);
+  two_sum(v2.x, -v0.x, f.e2.x, e2l.x);
+  two_sum(v2.y, -v0.y, f.e2.y, e2l.y);
+  two_sum(v2.z, -v0.z, f.e2.z, e2l.z);
+  cross2(f.e1, e1l, f.e2, e2l, f.Nh, f.Nl);
+  f.den = degenerate ? -1.f : dot(f.Nh, f.Nh);
+  return f;
+}
+
+// The plane hit of a ray from o along d on the face with constants f and first vertex v0.
+GVV_HD BaryGradSetup bary_grad_pixel(const BaryFace& f, V3 o, V3 d, V3 v0) {
   BaryGradSetup g;
   g.live = false;
-  {
-    // the reference's binary32 early exit (a decision)
-    const V3 e1 = v1 - v0, e2 = v2 - v0;
-    const V3 N = cross(e1, e2);
-    const float denf = dot(N, N);
-    if (fabsf(dot(normalize(d), normalize(N))) < 0.001f || fabsf(denf * denf) < 0.001f) return g;
-  }
+  // the reference's binary32 early exit (a decision): grazing ray or degenerate face
+  if (f.den < 0.f || fabsf(dot(normalize(d), f.nN)) < 0.001f) return g;
   g.live = true;
-  // the edges exactly (e = eh + el) and their cross product to twice the working precision:
-  // the plane's tilt enters the hit as |q0| |dN| / (|N| cos), so N must be far more accurate
-  // than binary32 for the hit to be accurate to binary32
-  V3 e1l, e2l;
-  two_sum(v1.x, -v0.x, g.e1.x, e1l.x);
-  two_sum(v1.y, -v0.y, g.e1.y, e1l.y);
-  two_sum(v1.z, -v0.z, g.e1.z, e1l.z);
-  two_sum(v2.x, -v0.x, g.e2.x, e2l.x);
-  two_sum(v2.y, -v0.y, g.e2.y, e2l.y);
-  two_sum(v2.z, -v0.z, g.e2.z, e2l.z);
-  V3 Nl;
-  cross2(g.e1, e1l, g.e2, e2l, g.N, Nl);
-  g.den = dot(g.N, g.N);
+  g.e1 = f.e1;
+  g.e2 = f.e2;
+  g.N = f.Nh;
+  g.den = f.den;
+  const V3 Nl = f.Nl;
   // w = v0 - o exactly, as wh + wl
   V3 wh, wl;
   two_sum(v0.x, -o.x, wh.x, wl.x);
@@ -143,6 +163,20 @@ GVV_HD BaryGradSetup bary_grad_setup(V3 o, V3 d, V3 v0, V3 v1, V3 v2) {
   g.nd = dh + dl;
   return g;
 }
+
+// The plane hit and the triangle constants of the Jacobian for a ray from o along d.
+GVV_HD BaryGradSetup bary_grad_setup(V3 o, V3 d, V3 v0, V3 v1, V3 v2) {
+  return bary_grad_pixel(bary_face_setup(v0, v1, v2), o, d, v0);
+}
+
+// Everything the backward's pixel kernel needs of one face besides its vertex ids: the scaled
+// ray/triangle constants that give the barycentrics, and the Jacobian's face constants (one
+// 128-byte line per face).
+struct alignas(16) FaceGrad {
+  TriSetup t;
+  BaryFace g;
+};
+static_assert(sizeof(FaceGrad) == 128, "one 128-byte line per face");
 
 // Arithmetic of continuous gradient terms.  Binary32 rounding is part of the renderer's
 // contract only for the visibility and decision values (barycentrics, texel indices, the

@@ -172,7 +172,7 @@ int plan_backward_into(Handle* h, int B, int texH, int texW, const float* g_rend
                        const float* target, const float* vpos, const float* vcol, const float* tex, const float* sh,
                        const float* vnormal, const float* bary, const int32_t* face, const float* extr,
                        const float* intr, float* g_pos, float* g_col, float* g_tex, float* g_sh, Camera* cams,
-                       float* g_vn, float* sh_partial, BackwardPlan& p) {
+                       float* g_pv, float* sh_partial, FaceGrad* face_grad, BackwardPlan& p) {
   if (B <= 0) return fail("batch size must be > 0");
   if (texH <= 0 || texW <= 0) return fail("texture height/width must be > 0");
   if (!g_pos || !g_col || !g_tex || !g_sh) return fail("null gradient output pointer");
@@ -190,16 +190,17 @@ int plan_backward_into(Handle* h, int B, int texH, int texW, const float* g_rend
   a.image_fs = h->image_fs; a.tile = h->tile; a.tiles_x = tx; a.tiles_y = ty;
   a.g_render = g_render; a.g_target = g_target; a.target = target;
   a.vpos = vpos; a.vcol = vcol; a.tex = tex; a.sh = sh; a.vnormal = vnormal; a.bary = bary; a.face = face;
-  a.faces = h->d_faces.p; a.uv = h->d_uv.p; a.cams = cams;
-  a.g_pos = g_pos; a.g_col = g_col; a.g_tex = g_tex; a.g_vn = g_vn; a.sh_partial = sh_partial; a.g_sh = g_sh;
+  a.faces = h->d_faces.p; a.uv = h->d_uv.p; a.cams = cams; a.face_grad = face_grad;
+  a.g_pos = g_pos; a.g_col = g_col; a.g_tex = g_tex; a.g_pv = g_pv; a.sh_partial = sh_partial; a.g_sh = g_sh;
   return 0;
 }
 
 BackwardScratch backward_scratch(const Handle* h, int B) {
   BackwardScratch s;
   s.cams = (size_t)B * h->C;
-  s.g_vn = (size_t)B * h->N * 3;
+  s.g_pv = (size_t)B * h->N * 6;
   s.sh_partial = (size_t)B * h->C * tiles(h->W, h->tile) * tiles(h->H, h->tile) * 27;
+  s.face_grad = (size_t)B * h->F;
   return s;
 }
 
@@ -210,11 +211,13 @@ int plan_backward(Handle* h, int B, int texH, int texW, const float* g_render, c
   if (B > 0) {
     const BackwardScratch n = backward_scratch(h, B);
     GVV_CHECK(h->cams.reserve(n.cams));
-    GVV_CHECK(h->g_vn.reserve(n.g_vn));
+    GVV_CHECK(h->g_pv.reserve(n.g_pv));
     GVV_CHECK(h->sh_partial.reserve(n.sh_partial));
+    GVV_CHECK(h->face_grad.reserve(n.face_grad));
   }
   return plan_backward_into(h, B, texH, texW, g_render, g_target, target, vpos, vcol, tex, sh, vnormal, bary, face,
-                            extr, intr, g_pos, g_col, g_tex, g_sh, h->cams.p, h->g_vn.p, h->sh_partial.p, p);
+                            extr, intr, g_pos, g_col, g_tex, g_sh, h->cams.p, h->g_pv.p, h->sh_partial.p,
+                            h->face_grad.p, p);
 }
 
 int check_device(const Handle* h) {

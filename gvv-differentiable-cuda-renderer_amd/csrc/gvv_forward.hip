@@ -48,7 +48,13 @@ __device__ __forceinline__ void setup_body(const SetupGrid& g, int blk) {
       float* o = a.fnormal + 3 * i;
       o[0] = n.x; o[1] = n.y; o[2] = n.z;
     }
-    a.tris[i] = tri_setup(v0, v1, v2);
+    if (a.tris) a.tris[i] = tri_setup(v0, v1, v2);
+    if (a.face_grad) {
+      FaceGrad r;
+      r.t = tri_setup(v0, v1, v2);
+      r.g = bary_face_setup(v0, v1, v2);
+      a.face_grad[i] = r;
+    }
     return;
   }
   blk -= g.face_blocks;

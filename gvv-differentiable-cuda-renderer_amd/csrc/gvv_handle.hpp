@@ -48,15 +48,17 @@ struct Handle {
   DevBuf<unsigned int> counters;
   DevBuf<int4> large;
   DevBuf<unsigned> tile_count, tile_dirty, bins;
-  DevBuf<float> g_vn;
+  DevBuf<float> g_pv;
   DevBuf<float> sh_partial;
+  DevBuf<FaceGrad> face_grad;          // the backward's per-face constants [B][F]
   Profiler prof;
   std::mutex mu;
 
   ~Handle() {
     d_faces.release(); d_csr_start.release(); d_csr_count.release(); d_csr_list.release();
     d_uv.release(); d_uv_table.release(); cams.release(); fnormal.release(); tris.release(); proj.release();
-    keys.release(); counters.release(); tile_count.release(); tile_dirty.release(); bins.release(); large.release(); g_vn.release(); sh_partial.release();
+    keys.release(); counters.release(); tile_count.release(); tile_dirty.release(); bins.release(); large.release(); g_pv.release(); sh_partial.release();
+    face_grad.release();
   }
 };
 
@@ -74,9 +76,9 @@ int plan_backward(Handle* h, int B, int texH, int texW, const float* g_render, c
                   const float* vnormal, const float* bary, const int32_t* face, const float* extr,
                   const float* intr, float* g_pos, float* g_col, float* g_tex, float* g_sh, BackwardPlan& p);
 
-// Element counts of the backward's scratch buffers for batch B: cameras, dL/d(vertex normal)
-// and the SH tile partials.
-struct BackwardScratch { size_t cams, g_vn, sh_partial; };
+// Element counts of the backward's scratch buffers for batch B: cameras, dL/d(vertex normal),
+// the SH tile partials and the per-face constants.
+struct BackwardScratch { size_t cams, g_pv, sh_partial, face_grad; };
 BackwardScratch backward_scratch(const Handle* h, int B);
 
 // plan_backward with caller-provided scratch (sized by backward_scratch): validates and fills
@@ -85,7 +87,7 @@ int plan_backward_into(Handle* h, int B, int texH, int texW, const float* g_rend
                        const float* target, const float* vpos, const float* vcol, const float* tex, const float* sh,
                        const float* vnormal, const float* bary, const int32_t* face, const float* extr,
                        const float* intr, float* g_pos, float* g_col, float* g_tex, float* g_sh, Camera* cams,
-                       float* g_vn, float* sh_partial, BackwardPlan& p);
+                       float* g_pv, float* sh_partial, FaceGrad* face_grad, BackwardPlan& p);
 
 // 0 when the current HIP device is the one the handle was created on; else 1 with the message set.
 int check_device(const Handle* h);

@@ -3,6 +3,7 @@
 #include <string>
 #include <vector>
 
+#include "gvv_bary_grad.hpp"
 #include "gvv_math.hpp"
 
 namespace gvv {
@@ -232,10 +233,12 @@ struct BackwardArgs {
   const int32_t* faces;
   const float* uv;
   const Camera* cams;
+  const FaceGrad* face_grad;         // [B][F] ray/triangle and Jacobian constants per face
   float* g_pos;                      // [B][N][3]
   float* g_col;                      // [B][N][3]
   float* g_tex;                      // [B][texH][texW][3]
-  float* g_vn;                       // [B][N][3] scratch: dL/d(vertex normal)
+  float* g_pv;                       // [B][N][6] scratch (shaded modes): per vertex the pixels'
+                                     // position terms (xyz), then dL/d(vertex normal) (xyz)
   float* sh_partial;                 // [B*C][tiles][27] scratch
   float* g_sh;                       // [B][C][27]
 };
@@ -245,11 +248,11 @@ struct BackwardArgs {
 struct FinishArgs {
   const float* vpos;
   const int32_t* faces;
-  const float* g_vn;
+  const float* g_pv;                 // [B][N][6]: position terms, normal gradient
   float* g_pos;
   const float* sh_partial;
   float* g_sh;
-  int B, N, F, tiles, fj_blocks;
+  int B, N, F, tiles, fj_blocks, pv_blocks;
 };
 FinishArgs finish_args(const BackwardArgs& a);
 
@@ -279,6 +282,7 @@ struct SetupArgs {
   const int32_t* faces = nullptr;
   float* fnormal = nullptr;
   TriSetup* tris = nullptr;
+  FaceGrad* face_grad = nullptr;     // the backward's per-face constants
   int B = 0, N = 0, F = 0;
   float* fill[kFillSegs] = {};
   int64_t fill_n[kFillSegs] = {};

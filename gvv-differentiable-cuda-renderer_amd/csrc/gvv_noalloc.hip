@@ -2,15 +2,16 @@
 // array, for callers that cannot allocate device memory (code under HIP graph capture, tight
 // serving loops, harnesses that forbid hipMalloc).  No counterpart in the reference.
 //
-// The backward's scratch -- per-view cameras, dL/d(vertex normal) and the SH tile partials --
-// lives in `scratch`, a file-scope device array of kScratchBytes that every GPU the library runs
+// The backward's scratch -- per-view cameras, dL/d(vertex normal), the SH tile partials and
+// the per-face constants -- lives in `scratch`, a file-scope device array of kScratchBytes that every GPU the library runs
 // on holds until the process ends.  In host code the array's name is only a host-side stand-in:
 // its device address is taken with hipGetSymbolAddress on the first call on each device (with
 // that device current), checked, and kept per device; only that address is passed to kernels.
 // Every call on one GPU shares that GPU's array, so calls that overlap on one GPU (different
 // streams, different host threads) race and must be serialised by the caller.  Everything the
-// kernels read from the array is written earlier in the same call: the cameras and the zero fill
-// of the normal gradients by the setup launch, every SH partial by the pixel kernel.
+// kernels read from the array is written earlier in the same call: the cameras, the per-face
+// constants and the zero fill of the normal gradients by the setup launch, every SH partial by
+// the pixel kernel.
 #include <mutex>
 #include <string>
 
@@ -73,8 +74,9 @@ extern "C" int gvv_backward_noalloc(void* handle, int32_t B, int32_t texH, int32
   }
   const BackwardScratch n = backward_scratch(h, B);
   const size_t off_vn = align256(n.cams * sizeof(Camera));
-  const size_t off_sh = off_vn + align256(n.g_vn * sizeof(float));
-  const size_t need = off_sh + n.sh_partial * sizeof(float);
+  const size_t off_sh = off_vn + align256(n.g_pv * sizeof(float));
+  const size_t off_face = off_sh + align256(n.sh_partial * sizeof(float));
+  const size_t need = off_face + n.face_grad * sizeof(FaceGrad);
   if (need > kScratchBytes)
     return set_last_error("gvv_backward_noalloc: batch " + std::to_string(B) + " needs " + std::to_string(need) +
                           " bytes of scratch, more than its fixed " + std::to_string(kScratchBytes) +
@@ -84,7 +86,8 @@ extern "C" int gvv_backward_noalloc(void* handle, int32_t B, int32_t texH, int32
   BackwardPlan p;
   if (plan_backward_into(h, B, texH, texW, g_render, g_target, target, vpos, vcol, tex, sh, vnormal, bary, face, extr,
                          intr, g_pos, g_col, g_tex, g_sh, reinterpret_cast<Camera*>(base),
-                         reinterpret_cast<float*>(base + off_vn), reinterpret_cast<float*>(base + off_sh), p))
+                         reinterpret_cast<float*>(base + off_vn), reinterpret_cast<float*>(base + off_sh),
+                         reinterpret_cast<FaceGrad*>(base + off_face), p))
     return 1;
   p.prof = nullptr;
   e = launch_backward(p, s);

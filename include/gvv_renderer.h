@@ -89,11 +89,13 @@ int gvv_backward(void* handle, int32_t B, int32_t texH, int32_t texW, const floa
 /* gvv_backward without any device allocation.  No counterpart in the reference: for callers
  * that cannot allocate device memory (code under HIP graph capture, tight serving loops,
  * harnesses that forbid hipMalloc).  Same arguments, results and zero-gradient modes as
- * gvv_backward.  The backward's scratch -- per-view cameras, the vertex-normal gradient and the
- * SH tile partials, B*C*156 + B*N*12 + B*C*ceil(W/tile)*ceil(H/tile)*108 bytes, each part
- * rounded up to 256 -- lives in one fixed 32 MiB device array per GPU that the library holds
- * until the process ends; a call that would need more is rejected with a message (use
- * gvv_backward or a smaller batch).  Every call
+ * gvv_backward.  The backward's scratch -- per-view cameras, the per-vertex position and
+ * normal terms, the SH tile partials and the per-face constants, B*C*156 + B*N*24
+ * + B*C*ceil(W/tile)*ceil(H/tile)*108 + B*F*128 bytes, each part rounded up to 256 -- lives in
+ * one fixed 32 MiB device array per GPU that the library holds until the process ends; a call
+ * that would need more is rejected with a message (use gvv_backward or a smaller batch).
+ * At config 1 (one mesh of 50k vertices and 100k faces, 8 views of 1024^2, tile 16) that is
+ * 17 MiB.  Every call
  * on one GPU shares that GPU's array: calls that overlap on one GPU (other streams, other host
  * threads, other handles) race and must be serialised by the caller. */
 int gvv_backward_noalloc(void* handle, int32_t B, int32_t texH, int32_t texW, const float* g_render,
