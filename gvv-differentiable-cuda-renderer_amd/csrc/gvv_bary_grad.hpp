@@ -197,29 +197,32 @@ GVV_HD F3 cross(F3 a, F3 b) { return f3(a.y * b.z - a.z * b.y, a.z * b.x - a.x *
 
 // Adds gb . d(a,b,c)/d(v0,v1,v2) (a 9-vector: vertex-major, xyz) to out[0..8].  inv_den and
 // inv_nd are reciprocals of g.den and g.nd (the caller's choice of reciprocal).
+//
+// The three vertices' terms share their cross products.  Every A_v x X is linear in A_v, and
+// A_0 = e1 - e2, A_1 = e2, A_2 = -e1, so the A x Cw, A x q0 (through dt_v) and A x N terms
+// of all three come from e1 x M and e2 x M with M = (Cw - s q0) / |N|^2 + c2 N, s = NEdw / N.d.
+// With K_r = q_r x N the edge and vertex terms collapse to Q_0 - D_0 = w1 K_2,
+// Q_1 - D_1 = -w0 K_2 and Q_2 - D_2 = w0 K_1 - w1 K_0 (q1 - E0 = q2, q2 - E1 = q0).
+// NEdw = N . (Ew x d) with Ew = w0 E0 + w1 E1.
 GVV_HD void bary_grad_apply(const BaryGradSetup& g, V3 dv, V3 gb, float inv_den, float inv_nd, float* out) {
   const F3 e1 = f3(g.e1), e2 = f3(g.e2), N = f3(g.N), q0 = f3(g.q0), d = f3(dv);
   const F3 q1 = q0 - e1, q2 = q0 - e2;
   const F3 E0 = e2 - e1, E1 = -e2;
   const float w0 = gb.x - gb.z, w1 = gb.y - gb.z;
-  const F3 C0 = cross(E0, q1), C1 = cross(E1, q2);
-  const F3 Cw = w0 * C0 + w1 * C1;
-  const float NCw = w0 * dot(N, C0) + w1 * dot(N, C1);
-  const float NEdw = w0 * dot(N, cross(E0, d)) + w1 * dot(N, cross(E1, d));
+  const F3 Cw = w0 * cross(E0, q1) + w1 * cross(E1, q2);
+  const float NCw = dot(N, Cw);
+  const float NEdw = dot(N, cross(w0 * E0 + w1 * E1, d));
   const float c2 = -2.f * NCw * inv_den * inv_den;
-#pragma unroll
-  for (int v = 0; v < 3; ++v) {
-    const F3 A = v == 0 ? e1 - e2 : (v == 1 ? e2 : -e1);
-    F3 dt = cross(A, q0) * (-inv_nd);
-    if (v == 0) dt = dt + N * inv_nd;
-    F3 Q = v == 0 ? cross(w1 * q2, N) : (v == 1 ? cross(-w0 * q1, N) : cross(w0 * q1 - w1 * q2, N));
-    if (v == 1) Q = Q - w0 * cross(N, E0);
-    if (v == 2) Q = Q - w1 * cross(N, E1);
-    const F3 gv = inv_den * (cross(A, Cw) + Q + NEdw * dt) + c2 * cross(A, N);
-    out[3 * v + 0] += gv.x;
-    out[3 * v + 1] += gv.y;
-    out[3 * v + 2] += gv.z;
-  }
+  const float s = NEdw * inv_nd;
+  const F3 M = inv_den * (Cw - s * q0) + c2 * N;
+  const F3 X1 = cross(e1, M), X2 = cross(e2, M);
+  const F3 K0 = cross(q0, N), K1 = cross(q1, N), K2 = cross(q2, N);
+  const F3 g0 = (X1 - X2) + inv_den * (w1 * K2 + s * N);
+  const F3 g1 = X2 - (inv_den * w0) * K2;
+  const F3 g2 = inv_den * (w0 * K1 - w1 * K0) - X1;
+  out[0] += g0.x; out[1] += g0.y; out[2] += g0.z;
+  out[3] += g1.x; out[4] += g1.y; out[5] += g1.z;
+  out[6] += g2.x; out[7] += g2.y; out[8] += g2.z;
 }
 #pragma clang fp contract(off)
 
