@@ -228,7 +228,7 @@ def fit_experiment(train_samples):
     cfg = fit.FitConfig()
     model = fit.ShapeModel(cfg, dev)
     net = proposal.ProposalNet(dev, seed=0)
-    log = proposal.train(net, model, samples=train_samples, batch=16, lr=7e-4, seed=1)
+    log = proposal.train(net, model, samples=train_samples, batch=16, lr=7e-4, seed=1, epochs=proposal.TRAIN_EPOCHS)
     gt = fit.sample_params(np.random.Generator(np.random.PCG64(12345)), cfg.batch, cfg.n_basis, dev)
     rep = proposal.compare_fits(net, model, gt)
     ratios = [r for r in rep.ratios if r is not None]
@@ -242,7 +242,7 @@ def fit_experiment(train_samples):
                 plain_final_loss=rep.plain_final_loss, plain_initial_loss=rep.plain_initial_loss,
                 warm_seconds=None if rep.warm_seconds is None else round(rep.warm_seconds, 4),
                 warm_iters=rep.warm_iters, warm_initial_loss=rep.warm_initial_loss, fallbacks=rep.fallbacks,
-                train_samples=log.samples, train_seconds=round(log.seconds, 2),
+                train_samples=log.samples, train_epochs=proposal.TRAIN_EPOCHS, train_seconds=round(log.seconds, 2),
                 config=dict(template_vertices=model.N, template_triangles=model.F, basis=cfg.n_basis,
                             subjects=cfg.batch, views=cfg.views, resolution=cfg.resolution, adam_lr=cfg.lr))
 

@@ -41,6 +41,7 @@ LAYERS = ((8, 32), (32, 64), (64, 128), (128, 128), (128, 256), (256, 256))   # 
 INPUT_RES = 256
 VIEW_RES = 128
 FEATURES = 0                      # input channels: 0 = RGB only; 1 = RGB - 0.5, coverage, tile x/y
+TRAIN_EPOCHS = 3                  # passes over the stored training renders
 
 
 def _ptr(t):
@@ -239,10 +240,16 @@ class TrainLog:
 
 
 def train(net: ProposalNet, model: fit.ShapeModel, samples: int = 200_000, batch: int = 16, lr: float = 7e-4,
-          seed: int = 1, log_every: int = 0, warmup: float = 0.02, weights=(1.0, 1.0, 1.0)) -> TrainLog:
-    """Trains on `samples` renders generated on the fly from the fit priors: Adam, linear warm-up
-    over the first `warmup` fraction of the steps then cosine decay, weighted MSE on normalised
-    coefficients (`weights` for the shape, SH and pose groups)."""
+          seed: int = 1, log_every: int = 0, warmup: float = 0.02, weights=(1.0, 1.0, 1.0),
+          epochs: int = 1) -> TrainLog:
+    """Trains on `samples` renders generated from the fit priors: Adam, linear warm-up over the
+    first `warmup` fraction of the steps then cosine decay, weighted MSE on normalised
+    coefficients (`weights` for the shape, SH and pose groups).
+
+    epochs = 1: each render is generated on the fly and used once.  epochs > 1: the renders'
+    RGB channels (bf16, 384 KiB per render at the fit's 4 views) and normalised coefficients
+    are generated once and kept in device memory, and every epoch visits them all in a seeded
+    random order of batches (RGB input only, features = 0)."""
     dev = model.device
     K = model.cfg.n_basis
     mean, std = param_stats(K, dev)
@@ -250,7 +257,12 @@ def train(net: ProposalNet, model: fit.ShapeModel, samples: int = 200_000, batch
     w[:K], w[K:K + 27], w[K + 27:] = weights[0], weights[1], weights[2]
     w = w * ((K + 33) / w.sum())
     opt = torch.optim.Adam(net.parameters(), lr=lr)
-    steps = max(1, samples // batch)
+    nb = max(1, samples // batch)
+    if epochs < 1:
+        raise ValueError("epochs must be >= 1")
+    if epochs > 1 and net.features != 0:
+        raise ValueError("training over stored renders keeps RGB only: use features=0")
+    steps = nb * epochs
     n_warm = max(1, int(warmup * steps))
 
     def factor(step):
@@ -262,10 +274,8 @@ def train(net: ProposalNet, model: fit.ShapeModel, samples: int = 200_000, batch
     rng = np.random.Generator(np.random.PCG64(seed))
     losses = []
     t0 = time.perf_counter()
-    for step in range(steps):
-        p = fit.sample_params(rng, batch, K, dev)
-        x = render_inputs(model, p, net.features)
-        target = (p.flat() - mean) / std
+
+    def sgd_step(step, x, target):
         out = net.forward(x)
         diff = out - target
         loss = (diff * diff).mean()
@@ -278,8 +288,29 @@ def train(net: ProposalNet, model: fit.ShapeModel, samples: int = 200_000, batch
         if log_every and (step % log_every == 0 or step == steps - 1):
             losses.append((step, float(loss)))
             print(f"proposal step {step}/{steps} loss {float(loss):.4f} {time.perf_counter() - t0:.1f}s", flush=True)
+
+    if epochs == 1:
+        for step in range(steps):
+            p = fit.sample_params(rng, batch, K, dev)
+            sgd_step(step, render_inputs(model, p, net.features), (p.flat() - mean) / std)
+    else:
+        X = torch.empty((nb, batch, INPUT_RES, INPUT_RES, 3), dtype=torch.bfloat16, device=dev)
+        Y = torch.empty((nb, batch, K + 33), dtype=torch.float32, device=dev)
+        for i in range(nb):
+            p = fit.sample_params(rng, batch, K, dev)
+            X[i] = render_inputs(model, p, 0)[..., :3]
+            Y[i] = (p.flat() - mean) / std
+        x = torch.zeros((batch, INPUT_RES, INPUT_RES, 8), dtype=torch.bfloat16, device=dev)
+        order = np.random.Generator(np.random.PCG64(seed + 1))
+        step = 0
+        for _ in range(epochs):
+            for i in order.permutation(nb):
+                x[..., :3] = X[int(i)]
+                sgd_step(step, x, Y[int(i)])
+                step += 1
+        del X, Y
     torch.cuda.synchronize()
-    return TrainLog(samples=steps * batch, seconds=time.perf_counter() - t0, losses=losses)
+    return TrainLog(samples=nb * batch, seconds=time.perf_counter() - t0, losses=losses)
 
 
 def propose(net: ProposalNet, model: fit.ShapeModel, target_render) -> fit.Params:

@@ -89,6 +89,22 @@ def test_training_beats_the_prior_mean(trained):
     assert math.isfinite(last) and last < 0.9
 
 
+def test_training_over_stored_renders(device):
+    """epochs > 1: the renders are generated once, kept in device memory and visited every
+    epoch; the run takes epochs x batches steps and learns as the on-the-fly run does."""
+    model = small_model(device)
+    net = proposal.ProposalNet(device, seed=0)
+    log = proposal.train(net, model, samples=32 * 100, batch=32, lr=1e-3, seed=5, log_every=1, epochs=3)
+    steps = [s for s, _ in log.losses]
+    assert steps[-1] == 299 and log.samples == 32 * 100
+    first = np.mean([l for _, l in log.losses[:5]])
+    last = np.mean([l for _, l in log.losses[-20:]])
+    print(f"stored renders, 3 epochs: loss {first:.4f} -> {last:.4f} in {log.seconds:.1f}s", flush=True)
+    assert math.isfinite(last) and last < 0.9
+    with pytest.raises(ValueError):
+        proposal.train(proposal.ProposalNet(device, seed=0, features=1), model, samples=64, batch=32, epochs=2)
+
+
 def test_trained_network_gains_over_the_plain_fit(device, trained):
     """What the trained network gains over the method without it, on held-out subjects (a
     seed the training never drew): the warm start begins below the plain start's loss, and the
