@@ -384,6 +384,30 @@ def stress_soup(seed, n_faces=80, width=64, height=48, albedo_mode="vertexColor"
                  width=width, height=height, albedo_mode=albedo_mode, meta=dict(name=f"stress{seed}"))
 
 
+def behind_camera(with_near=True, width=48, height=48):
+    """A triangle with one vertex 0.1 m behind the camera, in the plane 1 m off the optical axis,
+    its other two vertices 4 m ahead and outside the image on both sides; with `with_near`, also
+    a large triangle 0.01 mm in front of the camera covering the whole image (face 1).  The
+    geometry is given in camera coordinates (mm) and mapped to world coordinates."""
+    E = look_at((0.0, 0.0, -2.0))
+    K = pinhole(width, height, 1.0)
+    R, t = E[:, :3].astype(np.float64), E[:, 3].astype(np.float64)
+    world = lambda cam_pts: (R.T @ (np.asarray(cam_pts, np.float64) - t).T).T
+    tris = [world([[0.0, 1000.0, -100.0], [-3000.0, 1000.0, 4000.0], [3000.0, 1000.0, 4000.0]])]
+    if with_near:
+        tris.append(world([[-10.0, -10.0, 0.01], [10.0, -10.0, 0.01], [0.0, 10.0, 0.01]]))
+    v = np.concatenate(tris).astype(np.float32)
+    n = len(tris)
+    rng = np.random.Generator(np.random.PCG64(3))
+    return Scene(faces=np.arange(3 * n, dtype=np.int32).reshape(n, 3),
+                 uv=rng.uniform(0, 1, size=(n, 3, 2)).astype(np.float32), vpos=v[None],
+                 vcol=rng.uniform(0, 1, size=(1, 3 * n, 3)).astype(np.float32),
+                 tex=rng.uniform(0, 1, size=(1, 8, 8, 3)).astype(np.float32), sh=sh_coefficients(1, rng)[None],
+                 extr=E.reshape(1, 1, 12), intr=K.reshape(1, 1, 9),
+                 target=rng.uniform(0, 1, size=(1, 1, height, width, 3)).astype(np.float32),
+                 width=width, height=height, meta=dict(name="behind_camera"))
+
+
 # SH preset of the reference's demo scripts (python/data/test_SH_tensor.py:4): per channel
 # 0.7 ambient and -0.5 on the x band.
 REFERENCE_SH_PRESET = np.array([0.7, 0, 0, -0.5, 0, 0, 0, 0, 0] * 3, np.float32)

@@ -73,6 +73,15 @@ def test_stress_visibility(device, seed):
     _run(s, device, "vertexColor" if seed % 2 == 0 else "textured", "shaded", m2d=seed == 3, check_grads=seed < 2)
 
 
+@pytest.mark.parametrize("with_near", [False, True])
+def test_vertex_behind_the_camera(device, with_near):
+    """A vertex behind the camera (clamped projection depth) and a triangle 0.01 mm in front of
+    the camera: the oracle pin's scene (test_oracle_pins.py), on the GPU.  Without the
+    model-to-data term: its projection Jacobian divides by the near triangle's camera depth,
+    0.01 mm formed as the difference of 2 m coordinates, which binary32 holds to about 2%."""
+    _run(scenes.behind_camera(with_near), device, "textured", "shaded", m2d=False)
+
+
 @pytest.mark.parametrize("tuning", [(8, 64), (32, 1024), (8, 16), (32, 256)])
 def test_tuning_settings(device, tuning):
     """Every backward tile edge and large-face threshold the autotuner may pick keeps the

@@ -114,6 +114,76 @@ def test_nearer_triangle_wins():
         assert out["face"][0, 0, H // 2, W // 2] == near_id
 
 
+def _ray_hits(E, K, tri, px, py):
+    """FP64 barycentrics of the hit of pixel (px, py)'s ray on triangle tri, or None when the
+    ray misses its plane in front of the camera."""
+    R, t = E[:, :3], E[:, 3]
+    c = -R.T @ t
+    d = R.T @ np.linalg.solve(K, np.array([px, py, 1.0]))
+    n = np.cross(tri[1] - tri[0], tri[2] - tri[0])
+    den = d @ n
+    if abs(den) < 1e-12:
+        return None
+    lam = ((tri[0] - c) @ n) / den
+    if lam <= 0:
+        return None
+    X = c + lam * d
+    T = np.stack([tri[0] - tri[2], tri[1] - tri[2]], axis=1)
+    ab = np.linalg.lstsq(T, X - tri[2], rcond=None)[0]
+    return np.array([ab[0], ab[1], 1.0 - ab[0] - ab[1]])
+
+
+def test_vertex_behind_the_camera():
+    """A triangle with one vertex 0.1 m behind the camera (the reference clamps that vertex's
+    camera depth to 1e-5 before the perspective divide, the numerators K c keeping the unclamped
+    depth, CameraUtil.h:157-169).  Two consequences that follow from the geometry and fail for
+    plausible mistakes:
+    (a) coverage: the clamped vertex projects to (f x + cx z, f y + cy z) / 1e-5, far out
+    towards the image's bottom-left here, so the bounding box reaches the image border on that
+    side and the covered pixels are exactly the pixels whose rays hit the triangle in front of
+    the camera (with the depth used unclamped, negative, the vertex lands above the image and
+    the box keeps one row of them);
+    (b) depth: the clamped vertex's inverse depth 1e5 dominates the interpolated depth, so the
+    triangle wins over a triangle 0.01 mm in front of the camera, although that one is nearer
+    along every ray (a clamp to 1e-2 instead would make the near triangle win)."""
+    both = scenes.behind_camera()
+    W, H = both.width, both.height
+    E64, K64 = _camera_matrices(both)
+    v = both.vpos[0].astype(np.float64)
+    tri, near = v[:3], v[3:]
+
+    def render(n):
+        sc = scenes.behind_camera(with_near=n == 2)
+        return O.Oracle(**sc.renderer_kwargs()).forward(*sc.inputs())["face"][0, 0]
+
+    f = render(1)
+    inside = 0
+    for py in range(H):
+        for px in range(W):
+            abc = _ray_hits(E64, K64, tri, px + 0.5, py + 0.5)
+            if abc is not None and abc.min() > 1e-6:
+                assert f[py, px] == 0, (px, py, abc)
+                inside += 1
+            elif abc is None or abc.min() < -1e-3:
+                assert f[py, px] == -1, (px, py)
+    assert inside > 300
+
+    f2 = render(2)
+    won = lost = 0
+    for py in range(H):
+        for px in range(W):
+            t_abc = _ray_hits(E64, K64, tri, px + 0.5, py + 0.5)
+            n_abc = _ray_hits(E64, K64, near, px + 0.5, py + 0.5)
+            assert n_abc is not None and n_abc.min() > 1e-3
+            if t_abc is not None and t_abc.min() > 1e-6 and t_abc[0] > 1.5e-4:
+                assert f2[py, px] == 0, (px, py, t_abc)
+                won += 1
+            elif t_abc is None or t_abc.min() < -1e-3:
+                assert f2[py, px] == 1, (px, py)
+                lost += 1
+    assert won > 300 and lost > 300
+
+
 def _brute_force_ids(scene):
     """Every triangle at every pixel, smallest (depth key, id); independent of bounding boxes."""
     B, C = scene.B, scene.C
