@@ -159,6 +159,12 @@ int plan_forward(Handle* h, int B, int texH, int texW, const float* vpos, const 
   p.marks.bin_cap = bin_cap;
   p.vnormal_out = vnormal;
   p.normal_map_out = normal_map;
+  if (!h->side) {
+    GVV_CHECK(hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking));
+    GVV_CHECK(hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming));
+    GVV_CHECK(hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming));
+  }
+  p.side = h->side; p.fork = h->ev_fork; p.join = h->ev_join;
   ResolveArgs& r = p.r;
   r.B = B; r.C = h->C; r.N = h->N; r.W = h->W; r.H = h->H; r.texW = texW; r.texH = texH;
   r.keys = h->keys.p; r.vpos = vpos; r.vcol = vcol; r.tex = tex; r.sh = sh; r.uv = h->d_uv.p;

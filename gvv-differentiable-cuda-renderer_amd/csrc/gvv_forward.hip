@@ -348,6 +348,7 @@ __device__ __forceinline__ void raster_body(const RasterArgs& ra, int64_t wg) {
   }
   // large faces go to the queue with one atomic per wavefront (the cursor is a single word
   // that every wavefront of the launch would otherwise hit once per face)
+  if (!ra.queue_large) to_queue = false;
   const uint64_t qm = __ballot(to_queue);
   if (qm) {
     unsigned int base = 0;
@@ -398,6 +399,35 @@ __device__ __forceinline__ void raster_body(const RasterArgs& ra, int64_t wg) {
 }
 
 __global__ void __launch_bounds__(64) k_raster(RasterArgs ra) { raster_body(ra, xcd_contiguous(blockIdx.x, gridDim.x)); }
+
+// The depth pass's large-face test alone, so that the large-face pass can run beside the depth
+// pass: the (view, face) items whose bounding boxes exceed the threshold are appended to the
+// queue (one atomic per wavefront), by the same box computation as k_raster, which then skips
+// them.
+__global__ void __launch_bounds__(64) k_classify_large(RasterArgs ra) {
+  const int lane = threadIdx.x;
+  const int64_t i = (int64_t)blockIdx.x * 64 + lane;
+  bool to_queue = false;
+  int f = 0, bc = 0;
+  if (i < (int64_t)ra.B * ra.C * ra.F) {
+    f = (int)(i % ra.F);
+    bc = (int)(i / ra.F);
+    const RasterFace r = load_face(ra.faces, ra.proj + (size_t)bc * ra.N, f, ra.W, ra.H);
+    if (r.bx.x0 <= r.bx.x1 && r.bx.y0 <= r.bx.y1) {
+      const int bw = r.bx.x1 - r.bx.x0 + 1, bh = r.bx.y1 - r.bx.y0 + 1;
+      to_queue = (int64_t)bw * bh > ra.large_box;
+    }
+  }
+  const uint64_t qm = __ballot(to_queue);
+  if (!qm) return;
+  unsigned int base = 0;
+  if (lane == __ffsll((unsigned long long)qm) - 1) base = atomicAdd(ra.large_count, (unsigned)__popcll(qm));
+  base = __builtin_amdgcn_readlane(base, __ffsll((unsigned long long)qm) - 1);
+  if (to_queue) {
+    const unsigned off = __builtin_amdgcn_mbcnt_hi((unsigned)(qm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)qm, 0u));
+    ra.large[base + off] = make_int4(bc, f, ra.mesh_id, 0);
+  }
+}
 
 __global__ void __launch_bounds__(64) k_raster_multi(MeshTable<RasterArgs> t) {
   const MeshSlice m = t.find((unsigned)xcd_contiguous(blockIdx.x, gridDim.x));
@@ -645,6 +675,7 @@ RasterArgs raster_args(const ForwardPlan& p) {
   a.large_count = p.counters; a.large = p.large;
   a.B = p.B; a.C = p.C; a.N = p.N; a.F = p.F; a.W = p.r.W; a.H = p.r.H;
   a.large_box = p.large_box; a.mesh_id = p.mesh_id;
+  a.queue_large = 1;
   return a;
 }
 
@@ -684,6 +715,35 @@ hipError_t launch_forward_raster(const ForwardPlan& p, hipStream_t s) {
     int T = p.r.texW * p.r.texH;
     hipLaunchKernelGGL(k_normal_map, dim3(blocks_for((int64_t)B * T)), dim3(256), 0, s, p.uv_table, p.faces,
                        p.vnormal_out, p.normal_map_out, B, C, N, T);
+    return hipGetLastError();
+  }
+  // the large faces classified and rasterised on the handle's second stream, beside the depth
+  // pass (both take the 64-bit minimum of the same keys, so the order is free); in line when
+  // the caller's stream is being captured into a graph
+  hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
+  if (p.side && hipStreamIsCapturing(s, &capture) != hipSuccess) capture = hipStreamCaptureStatusActive;
+  if (!p.tiled && !p.shared_queue && p.side && capture == hipStreamCaptureStatusNone) {
+    hipError_t e = hipEventRecord(p.fork, s);
+    if (e == hipSuccess) e = hipStreamWaitEvent(p.side, p.fork, 0);
+    if (e != hipSuccess) return e;
+    RasterArgs ra = raster_args(p);
+    {
+      ProfScope ps(p.prof, "k_classify_large", p.side);
+      hipLaunchKernelGGL(k_classify_large, dim3(blocks_for((int64_t)B * C * F, 64)), dim3(64), 0, p.side, ra);
+    }
+    {
+      ProfScope ps(p.prof, "k_raster_large", p.side);
+      hipLaunchKernelGGL(k_raster_large, dim3(kLargeWorkgroups), dim3(64), 0, p.side, p.tris, p.faces, p.proj, p.cams,
+                         p.r.keys, p.counters, p.large, C, N, F, p.r.W, p.r.H, TileMarks{});
+    }
+    ra.queue_large = 0;
+    {
+      ProfScope ps(p.prof, "k_raster", s);
+      hipLaunchKernelGGL(k_raster, dim3(blocks_for((int64_t)B * C * F, 64)), dim3(64), 0, s, ra);
+    }
+    e = hipEventRecord(p.join, p.side);
+    if (e == hipSuccess) e = hipStreamWaitEvent(s, p.join, 0);
+    if (e != hipSuccess) return e;
     return hipGetLastError();
   }
   if (p.tiled) {

@@ -53,8 +53,16 @@ struct Handle {
   DevBuf<FaceGrad> face_grad;          // the backward's per-face constants [B][F]
   Profiler prof;
   std::mutex mu;
+  // the forward's second stream (default priority) for the large-face pass beside the depth
+  // pass, and its fork/join events; created at the first forward
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 
   ~Handle() {
+    if (side) (void)hipStreamSynchronize(side);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    if (side) (void)hipStreamDestroy(side);
     d_faces.release(); d_csr_start.release(); d_csr_count.release(); d_csr_list.release();
     d_uv.release(); d_uv_table.release(); cams.release(); fnormal.release(); tris.release(); proj.release();
     keys.release(); counters.release(); tile_count.release(); tile_dirty.release(); bins.release(); large.release(); g_pv.release(); sh_partial.release();

@@ -163,6 +163,8 @@ struct RasterArgs {
   unsigned int* large_count;         // queue length
   int4* large;                       // large-face queue
   int B, C, N, F, W, H, large_box, mesh_id;
+  int queue_large;                   // 1: append large faces to the queue; 0: skip them (queued
+                                     // already by k_classify_large)
 };
 
 struct ResolveArgs {
@@ -214,6 +216,10 @@ struct ForwardPlan {
   TileMarks marks;
   ResolveArgs r;
   Profiler* prof;
+  // the large-face pass beside the depth pass: a second stream of the handle and the events
+  // that fork it from the caller's stream after k_vertices and join it back (null: in line)
+  hipStream_t side;
+  hipEvent_t fork, join;
 };
 
 struct BackwardArgs {
