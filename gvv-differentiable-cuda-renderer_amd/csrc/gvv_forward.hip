@@ -566,7 +566,7 @@ __device__ __forceinline__ void resolve_body(const ResolveArgs& a, unsigned wg) 
     background_pixel(a, i);
     return;
   }
-  a.keys[i] = kEmptyKey;
+  st_out(a.keys + i, (unsigned long long)kEmptyKey);   // next read by the next call's depth pass
   shade_pixel<ALBEDO, SHADE>(a, bc, i, (int)(uint32_t)(key & 0xffffffffu), pixel_ray(a.cams[bc], px + 0.5f, py + 0.5f));
 }
 

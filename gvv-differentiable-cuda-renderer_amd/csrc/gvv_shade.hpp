@@ -7,11 +7,17 @@
 
 namespace gvv {
 
+// The pixel outputs are written once and not read again by the forward: streaming
+// (non-temporal) stores, which keep them from displacing the vertex, texel and key lines the
+// resolve reads (k_resolve 0.104 -> 0.090 ms at 8 x 1024^2).
+template <class T>
+__device__ __forceinline__ void st_out(T* p, T v) { __builtin_nontemporal_store(v, p); }
+
 // Background: face -1, barycentrics 0, render (0, 1, 0) (CUDABasedRasterization.cu:80-89).
 __device__ __forceinline__ void background_pixel(const ResolveArgs& a, int64_t i) {
-  a.face[i] = -1;
-  a.bary[2 * i] = 0.f; a.bary[2 * i + 1] = 0.f;
-  a.render[3 * i] = 0.f; a.render[3 * i + 1] = 1.f; a.render[3 * i + 2] = 0.f;
+  st_out(a.face + i, -1);
+  st_out(a.bary + 2 * i, 0.f); st_out(a.bary + 2 * i + 1, 0.f);
+  st_out(a.render + 3 * i, 0.f); st_out(a.render + 3 * i + 1, 1.f); st_out(a.render + 3 * i + 2, 0.f);
 }
 
 // Pixel i of view bc with visible face f and unit ray d through the pixel centre.
@@ -24,8 +30,8 @@ __device__ __forceinline__ void shade_pixel(const ResolveArgs& a, int bc, int64_
   const TriSetup t = tri_setup(ld3(vp + 3 * i0), ld3(vp + 3 * i1), ld3(vp + 3 * i2));
   const Camera& cam = a.cams[bc];
   const V3 abc = ray_barycentric(t, cam_origin_scaled(cam), d);
-  a.face[i] = f;
-  a.bary[2 * i] = abc.x; a.bary[2 * i + 1] = abc.y;
+  st_out(a.face + i, f);
+  st_out(a.bary + 2 * i, abc.x); st_out(a.bary + 2 * i + 1, abc.y);
 
   const float* vn = a.vnormal + (size_t)bc * a.N * 3;
   V3 n = pixel_normal(ld3(vn + 3 * i0), ld3(vn + 3 * i1), ld3(vn + 3 * i2), abc);
@@ -49,7 +55,7 @@ __device__ __forceinline__ void shade_pixel(const ResolveArgs& a, int bc, int64_
     V3 L = sh_illum(n, a.sh + (size_t)bc * 27);
     col = v3(L.x * col.x, L.y * col.y, L.z * col.z);
   }
-  a.render[3 * i] = col.x; a.render[3 * i + 1] = col.y; a.render[3 * i + 2] = col.z;
+  st_out(a.render + 3 * i, col.x); st_out(a.render + 3 * i + 1, col.y); st_out(a.render + 3 * i + 2, col.z);
 }
 
 }  // namespace gvv
