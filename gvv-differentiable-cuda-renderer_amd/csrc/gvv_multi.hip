@@ -30,7 +30,9 @@ namespace {
 
 using namespace gvv;
 
-constexpr int kLanes = 4;                       // graph branches: the hardware queues per process
+// graph branches: twice the process's 4 hardware queues (config 2: 7.75-7.84 ms per run against
+// 7.97-8.00 with 4 branches, 8.0-8.1 with 12 or 16, 9.2 with 2)
+constexpr int kLanes = 8;
 constexpr int kTiles[3] = {8, 16, 32};          // backward pixel-tile edges
 constexpr int kLargeBoxes[3] = {64, 256, 1024}; // raster large-face thresholds (pixels)
 
@@ -319,7 +321,7 @@ template <class Enqueue>
 hipError_t fan_out(Multi* m, const std::vector<int>& order, const gvv_mesh_io* io, int stage, Enqueue&& enqueue) {
   hipError_t e = hipEventRecord(m->fork[stage], m->lanes[0]);
   for (int k = 1; k < kLanes && e == hipSuccess; ++k) e = hipStreamWaitEvent(m->lanes[k], m->fork[stage], 0);
-  double load[kLanes] = {0.0, 0.0, 0.0, 0.0};
+  double load[kLanes] = {};
   for (size_t j = 0; j < order.size() && e == hipSuccess; ++j) {
     const int i = order[j];
     int lane = 0;
