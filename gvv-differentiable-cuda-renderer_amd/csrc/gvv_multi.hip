@@ -30,8 +30,8 @@ namespace {
 
 using namespace gvv;
 
-// graph branches: twice the process's 4 hardware queues (config 2: 7.75-7.84 ms per run against
-// 7.97-8.00 with 4 branches, 8.0-8.1 with 12 or 16, 9.2 with 2)
+// graph branches: twice the process's 4 hardware queues (32 uneven meshes, 103 Mpix per run:
+// 7.75-7.84 ms against 7.97-8.00 with 4 branches, 8.0-8.1 with 12 or 16, 9.2 with 2)
 constexpr int kLanes = 8;
 constexpr int kTiles[3] = {8, 16, 32};          // backward pixel-tile edges
 constexpr int kLargeBoxes[3] = {64, 256, 1024}; // raster large-face thresholds (pixels)

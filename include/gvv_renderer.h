@@ -64,7 +64,11 @@ void gvv_destroy(void* handle);
  *            otherwise).
  * The op's target_image_out output is the target input itself (an alias); it is not handled
  * here.  Visibility ties between faces at the same integer depth resolve to the lowest face
- * id (the reference resolves them by an unsynchronised last write). */
+ * id (the reference resolves them by an unsynchronised last write).
+ * Work is ordered on the caller's stream.  Outside stream capture, the large-face part of the
+ * depth pass runs on a second stream the handle owns, forked from and joined back to the
+ * caller's stream with events inside the call, so the call's results are complete in the
+ * caller's stream order as usual; under capture everything stays on the caller's stream. */
 int gvv_forward(void* handle, int32_t B, int32_t texH, int32_t texW, const float* vpos, const float* vcol,
                 const float* tex, const float* sh, const float* extr, const float* intr, float* bary,
                 int32_t* face, float* render, float* vnormal, float* normal_map, void* hip_stream);
@@ -94,8 +98,8 @@ int gvv_backward(void* handle, int32_t B, int32_t texH, int32_t texW, const floa
  * + B*C*ceil(W/tile)*ceil(H/tile)*108 + B*F*128 bytes, each part rounded up to 256 -- lives in
  * one fixed 32 MiB device array per GPU that the library holds until the process ends; a call
  * that would need more is rejected with a message (use gvv_backward or a smaller batch).
- * At config 1 (one mesh of 50k vertices and 100k faces, 8 views of 1024^2, tile 16) that is
- * 17 MiB.  Every call
+ * For one mesh of 50k vertices and 100k faces in 8 views of 1024^2 (tile 16) that is 17 MiB.
+ * Every call
  * on one GPU shares that GPU's array: calls that overlap on one GPU (other streams, other host
  * threads, other handles) race and must be serialised by the caller. */
 int gvv_backward_noalloc(void* handle, int32_t B, int32_t texH, int32_t texW, const float* g_render,
