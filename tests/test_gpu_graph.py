@@ -43,3 +43,25 @@ def test_forward_backward_graph_replays(device):
             e = rel_err(a.cpu().numpy(), b.cpu().numpy())
             assert e <= 1e-6, (k, name, e)
     print("3 replays equal the eager call", flush=True)
+
+
+def test_forward_on_null_and_user_streams(device):
+    """The forward's fork to the handle's second stream and join back work from the NULL
+    (legacy default) stream as from a user stream: identical results, and the outputs are
+    complete in the caller's stream order without a device-wide synchronisation."""
+    s = scenes.config1(width=256, height=256, cols=80, rows=60, n_cameras=2, tex_size=128)
+    ins = [torch.from_numpy(np.ascontiguousarray(a)).to(device) for a in s.inputs()]
+    outs = []
+    for stream in (torch.cuda.default_stream(device), torch.cuda.Stream(device)):
+        r = Rasterizer(**s.renderer_kwargs())
+        r.set_tuning(16, 64)                      # a low threshold: many faces take the large-face pass
+        with torch.cuda.stream(stream):
+            bary, face, render, _, _ = r.forward(*ins)
+            # consumed on the same stream before any synchronisation
+            checksum = (face.to(torch.int64).sum(), render.sum(), bary.sum())
+        stream.synchronize()
+        outs.append((face.clone(), render.clone(), bary.clone(), [float(c) for c in checksum]))
+    (f0, r0, b0, c0), (f1, r1, b1, c1) = outs
+    assert torch.equal(f0, f1) and torch.equal(r0, r1) and torch.equal(b0, b1)
+    assert c0 == c1
+    assert c0[0] == float(f0.to(torch.int64).sum())
