@@ -41,7 +41,7 @@ LAYERS = ((8, 32), (32, 64), (64, 128), (128, 128), (128, 256), (256, 256))   # 
 INPUT_RES = 256
 VIEW_RES = 128
 FEATURES = 0                      # input channels: 0 = RGB only; 1 = RGB - 0.5, coverage, tile x/y
-TRAIN_EPOCHS = 3                  # passes over the stored training renders
+TRAIN_EPOCHS = 4                  # passes over the stored training renders
 
 
 def _ptr(t):
