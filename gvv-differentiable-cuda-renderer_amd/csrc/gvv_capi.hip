@@ -159,12 +159,7 @@ int plan_forward(Handle* h, int B, int texH, int texW, const float* vpos, const 
   p.marks.bin_cap = bin_cap;
   p.vnormal_out = vnormal;
   p.normal_map_out = normal_map;
-  if (!h->side) {
-    GVV_CHECK(hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking));
-    GVV_CHECK(hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming));
-    GVV_CHECK(hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming));
-  }
-  p.side = h->side; p.fork = h->ev_fork; p.join = h->ev_join;
+  p.side = nullptr; p.fork = p.join = nullptr;      // the single-mesh entry point sets them
   ResolveArgs& r = p.r;
   r.B = B; r.C = h->C; r.N = h->N; r.W = h->W; r.H = h->H; r.texW = texW; r.texH = texH;
   r.keys = h->keys.p; r.vpos = vpos; r.vcol = vcol; r.tex = tex; r.sh = sh; r.uv = h->d_uv.p;
@@ -306,6 +301,14 @@ int gvv_forward(void* handle, int32_t B, int32_t texH, int32_t texW, const float
   ForwardPlan p;
   if (plan_forward(h, B, texH, texW, vpos, vcol, tex, sh, extr, intr, bary, face, render, vnormal, normal_map, s, p))
     return 1;
+  // the handle's second stream for the large-face pass, made at its first forward (the
+  // multi-mesh runtime's meshes share one queue and never use one)
+  if (!h->side) {
+    GVV_CHECK(hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming));
+    GVV_CHECK(hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming));
+    GVV_CHECK(hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking));
+  }
+  p.side = h->side; p.fork = h->ev_fork; p.join = h->ev_join;
   p.prof = h->prof.enabled ? &h->prof : nullptr;
   GVV_CHECK(launch_forward(p, s));
   return 0;

@@ -726,6 +726,7 @@ hipError_t launch_forward_raster(const ForwardPlan& p, hipStream_t s) {
     hipError_t e = hipEventRecord(p.fork, s);
     if (e == hipSuccess) e = hipStreamWaitEvent(p.side, p.fork, 0);
     if (e != hipSuccess) return e;
+    // from here on the second stream is joined back whatever fails in between
     RasterArgs ra = raster_args(p);
     {
       ProfScope ps(p.prof, "k_classify_large", p.side);
@@ -736,15 +737,17 @@ hipError_t launch_forward_raster(const ForwardPlan& p, hipStream_t s) {
       hipLaunchKernelGGL(k_raster_large, dim3(kLargeWorkgroups), dim3(64), 0, p.side, p.tris, p.faces, p.proj, p.cams,
                          p.r.keys, p.counters, p.large, C, N, F, p.r.W, p.r.H, TileMarks{});
     }
+    const hipError_t e_side = hipGetLastError();
     ra.queue_large = 0;
     {
       ProfScope ps(p.prof, "k_raster", s);
       hipLaunchKernelGGL(k_raster, dim3(blocks_for((int64_t)B * C * F, 64)), dim3(64), 0, s, ra);
     }
+    const hipError_t e_main = hipGetLastError();
     e = hipEventRecord(p.join, p.side);
     if (e == hipSuccess) e = hipStreamWaitEvent(s, p.join, 0);
     if (e != hipSuccess) return e;
-    return hipGetLastError();
+    return e_side != hipSuccess ? e_side : e_main;
   }
   if (p.tiled) {
     ProfScope ps(p.prof, "k_bin", s);
