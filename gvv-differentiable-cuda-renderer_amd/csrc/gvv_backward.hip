@@ -13,13 +13,14 @@
 //                  24-byte row of a scratch buffer, so that they cost one atomic request rather
 //                  than two.  SH terms accumulate in registers over the tile and are reduced
 //                  once per tile (a transposed DPP sum) into a partial buffer.
-//   k_grad_finish  one launch of three parts.  Face Jacobian: the normal -> position chain.
+//   k_grad_finish  one launch of two parts.  Position gradient: the normal -> position chain.
 //                  Instead of walking every incident face of every corner for every pixel
-//                  (Grad.cu:559-615), the pixels accumulate dL/d(vertex normal) per vertex; one
-//                  thread per face then forms G = sum of its vertices' normal gradients and
-//                  scatters G x (vj - vi), (vk - vi) x G and -(both) -- the same sum, regrouped
-//                  by face.  Position terms: the pixels' position terms from the scratch rows
-//                  into the position gradient.  SH: fixed-order reduction of the SH partials
+//                  (Grad.cu:559-615), the pixels accumulate dL/d(vertex normal) per vertex;
+//                  each vertex then gathers, over its incident faces, the face's share: with G
+//                  the sum of the face's vertices' normal gradients, G x (vj - vi) to corner k,
+//                  (vk - vi) x G to corner j and minus both to corner i -- the same sum,
+//                  regrouped -- and adds the pixels' position terms from the scratch rows;
+//                  plain stores, no atomics.  SH: fixed-order reduction of the SH partials
 //                  (deterministic).
 #include "gvv_bary_grad.hpp"
 #include "gvv_kernels.hpp"
@@ -521,78 +522,76 @@ __device__ __forceinline__ void sh_reduce_block(const float* __restrict__ partia
   if (threadIdx.x == 0) g_sh[(size_t)bc * 27 + k] = red[0];
 }
 
-// Normal -> position chain regrouped by face (see the file header), for the workgroup's 256
-// faces.
-__device__ __forceinline__ void face_jac_block(const float* __restrict__ vpos, const int32_t* __restrict__ faces,
-                                               const float* __restrict__ g_pv, float* __restrict__ g_pos, int B, int N,
-                                               int F, int blk, float* smem) {
-  // the workgroup's 256 faces x 3 corners x 3 channels are staged in LDS and added with lane q
-  // taking channel q mod 3 of corner q / 3, so that a vertex's three channels are adjacent
-  // lanes of one atomic instruction (atomics cost per cache line touched, not per lane)
-  int* s_vtx = reinterpret_cast<int*>(smem);            // [256 * 3] corner vertex ids (-1: nothing)
-  int* s_b = s_vtx + 256 * 3;                            // [256] batch entry of the face
-  float* s_val = reinterpret_cast<float*>(s_b + 256);    // [256 * 9]
+// Face f's share of vertex v's position gradient through the vertex normals: with G the sum
+// of the normal gradients of f's distinct vertices, G x (vj - vi) goes to corner k,
+// (vk - vi) x G to corner j and minus both to corner i; v takes the shares of every corner it
+// holds.
+__device__ __forceinline__ V3 normal_chain_share(const float* __restrict__ vp, const float* __restrict__ gv,
+                                                 const int32_t* __restrict__ faces, int f, int v) {
+  const int a0 = faces[3 * f], a1 = faces[3 * f + 1], a2 = faces[3 * f + 2];
+  V3 G = ld3(gv + 6 * a0);
+  if (a1 != a0) G = G + ld3(gv + 6 * a1);
+  if (a2 != a0 && a2 != a1) G = G + ld3(gv + 6 * a2);
+  V3 r = v3(0.f, 0.f, 0.f);
+  if (G.x == 0.f && G.y == 0.f && G.z == 0.f) return r;
+  const V3 vi = ld3(vp + 3 * a0), vj = ld3(vp + 3 * a1), vk = ld3(vp + 3 * a2);
+  const V3 gk = cross(G, vj - vi);
+  const V3 gj = cross(vk - vi, G);
+  if (a0 == v) r = r - (gj + gk);
+  if (a1 == v) r = r + gj;
+  if (a2 == v) r = r + gk;
+  return r;
+}
+
+__device__ __forceinline__ float wave_sum(float x) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) x += __shfl_xor(x, m);
+  return x;
+}
+
+// The position gradient of the workgroup's 256 (batch entry, vertex) items, gathered: the
+// pixels' position terms from the scratch rows plus every incident face's normal-chain share
+// (over the vertex -> face lists, in list order), written with plain stores -- no atomics.  A
+// vertex with more than 16 incident faces (the centre of a cap fan has 250) is gathered by
+// its whole wavefront.
+__device__ __forceinline__ void pos_gather_block(const FinishArgs& f, int blk) {
+  constexpr int kHeavy = 16;
   const int64_t i = (int64_t)blk * 256 + threadIdx.x;
-  const int t = threadIdx.x;
-  s_vtx[3 * t] = s_vtx[3 * t + 1] = s_vtx[3 * t + 2] = -1;
-  if (i < (int64_t)B * F) {
-    const int b = (int)(i / F);
-    s_b[t] = b;
-    const int f = (int)(i % F);
-    const int a0 = faces[3 * f], a1 = faces[3 * f + 1], a2 = faces[3 * f + 2];
-    const float* gv = g_pv + (size_t)b * N * 6 + 3;     // the normal-gradient half of each row
-    V3 G = ld3(gv + 6 * a0);
-    if (a1 != a0) G = G + ld3(gv + 6 * a1);
-    if (a2 != a0 && a2 != a1) G = G + ld3(gv + 6 * a2);
-    if (!(G.x == 0.f && G.y == 0.f && G.z == 0.f)) {
-      const float* vp = vpos + (size_t)b * N * 3;
-      const V3 vi = ld3(vp + 3 * a0), vj = ld3(vp + 3 * a1), vk = ld3(vp + 3 * a2);
-      const V3 gk = cross(G, vj - vi);
-      const V3 gj = cross(vk - vi, G);
-      const V3 gi = -(gj + gk);
-      s_vtx[3 * t] = a0; s_vtx[3 * t + 1] = a1; s_vtx[3 * t + 2] = a2;
-      float* v = s_val + 9 * t;
-      v[0] = gi.x; v[1] = gi.y; v[2] = gi.z;
-      v[3] = gj.x; v[4] = gj.y; v[5] = gj.z;
-      v[6] = gk.x; v[7] = gk.y; v[8] = gk.z;
-    }
+  const bool valid = i < (int64_t)f.B * f.N;
+  const int b = valid ? (int)(i / f.N) : 0, v = valid ? (int)(i % f.N) : 0;
+  const int s = valid ? f.csr_start[v] : 0, n = valid ? f.csr_count[v] : 0;
+  const float* vp = f.vpos + (size_t)b * f.N * 3;
+  const float* gv = f.g_pv + (size_t)b * f.N * 6 + 3;   // the normal-gradient part of each row
+  V3 acc = v3(0.f, 0.f, 0.f);
+  if (valid) acc = ld3(f.g_pv + (size_t)i * 6);
+  if (n <= kHeavy)
+    for (int k = 0; k < n; ++k) acc = acc + normal_chain_share(vp, gv, f.faces, f.csr_list[s + k], v);
+  const int lane = threadIdx.x & 63;
+  for (uint64_t heavy = __ballot(n > kHeavy); heavy; heavy &= heavy - 1) {
+    const int j = __ffsll((unsigned long long)heavy) - 1;
+    const int sj = __builtin_amdgcn_readlane(s, j), nj = __builtin_amdgcn_readlane(n, j);
+    const int vj = __builtin_amdgcn_readlane(v, j), bj = __builtin_amdgcn_readlane(b, j);
+    const float* vpj = f.vpos + (size_t)bj * f.N * 3;
+    const float* gvj = f.g_pv + (size_t)bj * f.N * 6 + 3;
+    V3 part = v3(0.f, 0.f, 0.f);
+    for (int k = lane; k < nj; k += 64) part = part + normal_chain_share(vpj, gvj, f.faces, f.csr_list[sj + k], vj);
+    const V3 tot = v3(wave_sum(part.x), wave_sum(part.y), wave_sum(part.z));
+    if (lane == j) acc = acc + tot;
   }
-  __syncthreads();
-  for (int q = t; q < 256 * 9; q += 256) {
-    const int slot = q / 9, r = q - 9 * slot;
-    const int vtx = s_vtx[3 * slot + r / 3];
-    if (vtx < 0) continue;
-    atomicAdd(g_pos + ((size_t)s_b[slot] * N + vtx) * 3 + r % 3, s_val[q]);
-  }
+  if (!valid) return;
+  float* o = f.g_pos + (size_t)i * 3;
+  o[0] = acc.x; o[1] = acc.y; o[2] = acc.z;
 }
 
-// The pixels' position terms of the workgroup's 768 position-gradient floats, added from the
-// interleaved rows (3 per thread, consecutive floats on consecutive lanes).
-__device__ __forceinline__ void pos_terms_block(const float* __restrict__ g_pv, float* __restrict__ g_pos,
-                                                int64_t n, int blk) {
-  for (int q = threadIdx.x; q < 768; q += 256) {
-    const int64_t i = (int64_t)blk * 768 + q;
-    if (i >= n) return;
-    const float v = g_pv[(i / 3) * 6 + i % 3];
-    if (v != 0.f) atomicAdd(g_pos + i, v);
-  }
-}
-
-// The reductions after the pixel kernel in one launch: blocks [0, fj_blocks) run the face
-// Jacobian, the next pv_blocks add the pixels' position terms, the rest run the SH reduction
-// (one per view and coefficient).  The first two add into the zero-filled position gradient.
+// The reductions after the pixel kernel in one launch: blocks [0, vg_blocks) gather the
+// position gradient, the rest run the SH reduction (one per view and coefficient).
 __device__ __forceinline__ void grad_finish_body(const FinishArgs& f, int blk) {
-  __shared__ float smem[256 * 13];
-  if (blk < f.fj_blocks) {
-    face_jac_block(f.vpos, f.faces, f.g_pv, f.g_pos, f.B, f.N, f.F, blk, smem);
+  __shared__ float smem[256];
+  if (blk < f.vg_blocks) {
+    pos_gather_block(f, blk);
     return;
   }
-  blk -= f.fj_blocks;
-  if (blk < f.pv_blocks) {
-    pos_terms_block(f.g_pv, f.g_pos, (int64_t)f.B * f.N * 3, blk);
-    return;
-  }
-  sh_reduce_block(f.sh_partial, f.g_sh, f.tiles, blk - f.pv_blocks, smem);
+  sh_reduce_block(f.sh_partial, f.g_sh, f.tiles, blk - f.vg_blocks, smem);
 }
 
 __global__ void __launch_bounds__(256) k_grad_finish(FinishArgs f) { grad_finish_body(f, blockIdx.x); }
@@ -606,9 +605,9 @@ __global__ void __launch_bounds__(256) k_grad_finish_multi(MeshTable<FinishArgs>
 FinishArgs finish_args(const BackwardArgs& a) {
   FinishArgs f;
   f.vpos = a.vpos; f.faces = a.faces; f.g_pv = a.g_pv; f.g_pos = a.g_pos; f.sh_partial = a.sh_partial; f.g_sh = a.g_sh;
+  f.csr_start = a.csr_start; f.csr_count = a.csr_count; f.csr_list = a.csr_list;
   f.B = a.B; f.N = a.N; f.F = a.F; f.tiles = a.tiles_x * a.tiles_y;
-  f.fj_blocks = (int)(((int64_t)a.B * a.F + 255) / 256);
-  f.pv_blocks = (int)(((int64_t)a.B * a.N * 3 + 767) / 768);
+  f.vg_blocks = (int)(((int64_t)a.B * a.N + 255) / 256);
   return f;
 }
 
@@ -631,7 +630,7 @@ static void launch_pixel_grad_m(const BackwardArgs& a, hipStream_t s) {
 
 static inline unsigned blocks_for(int64_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
 
-unsigned finish_blocks(const FinishArgs& f, int C) { return (unsigned)(f.fj_blocks + f.pv_blocks + f.B * C * 27); }
+unsigned finish_blocks(const FinishArgs& f, int C) { return (unsigned)(f.vg_blocks + f.B * C * 27); }
 
 hipError_t launch_grad_finish_multi(const MeshTable<FinishArgs>& t, unsigned blocks, hipStream_t s) {
   if (blocks == 0) return hipSuccess;

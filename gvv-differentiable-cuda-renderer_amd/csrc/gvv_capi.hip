@@ -192,6 +192,7 @@ int plan_backward_into(Handle* h, int B, int texH, int texW, const float* g_rend
   a.g_render = g_render; a.g_target = g_target; a.target = target;
   a.vpos = vpos; a.vcol = vcol; a.tex = tex; a.sh = sh; a.vnormal = vnormal; a.bary = bary; a.face = face;
   a.faces = h->d_faces.p; a.uv = h->d_uv.p; a.cams = cams; a.face_grad = face_grad;
+  a.csr_start = h->d_csr_start.p; a.csr_count = h->d_csr_count.p; a.csr_list = h->d_csr_list.p;
   a.g_pos = g_pos; a.g_col = g_col; a.g_tex = g_tex; a.g_pv = g_pv; a.sh_partial = sh_partial; a.g_sh = g_sh;
   return 0;
 }

@@ -239,6 +239,9 @@ struct BackwardArgs {
   const int32_t* faces;
   const float* uv;
   const Camera* cams;
+  const int32_t* csr_start;          // vertex -> incident faces (the finishing gather)
+  const int32_t* csr_count;
+  const int32_t* csr_list;
   const FaceGrad* face_grad;         // [B][F] ray/triangle and Jacobian constants per face
   float* g_pos;                      // [B][N][3]
   float* g_col;                      // [B][N][3]
@@ -254,11 +257,14 @@ struct BackwardArgs {
 struct FinishArgs {
   const float* vpos;
   const int32_t* faces;
+  const int32_t* csr_start;
+  const int32_t* csr_count;
+  const int32_t* csr_list;
   const float* g_pv;                 // [B][N][6]: position terms, normal gradient
   float* g_pos;
   const float* sh_partial;
   float* g_sh;
-  int B, N, F, tiles, fj_blocks, pv_blocks;
+  int B, N, F, tiles, vg_blocks;
 };
 FinishArgs finish_args(const BackwardArgs& a);
 
