@@ -385,6 +385,11 @@ def main():
         torch.cuda.synchronize()
         tuning_ms[(tile, box)] = a.elapsed_time(z) / 3
     best = min(tuning_ms, key=tuning_ms.get)
+    # the library's defaults (tile 16, large box 256) stay unless a candidate beats them by more
+    # than the 3-step trials' noise (about 1%): a toss-up then runs at the settings the committed
+    # counter profiles were recorded at
+    if (16, 256) in tuning_ms and tuning_ms[best] > 0.99 * tuning_ms[(16, 256)]:
+        best = (16, 256)
     rast.set_tuning(*best)
 
     for _ in range(args.warmup):
