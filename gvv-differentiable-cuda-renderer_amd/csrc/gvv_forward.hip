@@ -197,22 +197,41 @@ __device__ __forceinline__ RasterFace load_face(const int32_t* __restrict__ face
   return r;
 }
 
-// Per-face constants of the exact test, staged in LDS for the candidate pass.
-struct FaceStage {
-  TriSetup t;           // scaled vertices, N, v0.N, N.N
-  float z0, z1, z2;     // projected depths
-  int f, x0, y0, bw, bc;
+// Per-face constants of the exact test, staged in LDS for the candidate pass: 64 bytes, so
+// that 64 staged faces and the candidate list take 5 KB and eight wavefronts fit per SIMD.
+// N.N is recomputed per candidate; the face id, view and box stay in registers (StageOwner).
+// Packed as four 16-byte words: (v0, v1.x), (v1.yz, v2.xy), (v2.z, N), (v0.N, z0, z1, z2), with
+// v0..v2 and N the scaled vertices and face normal of the ray/triangle test (TriSetup) and z0..z2
+// the projected depths.
+struct alignas(16) FaceStage {
+  float4 q[4];
 };
 
-// Candidate list of one wavefront: 32-bit words naming a staged face (its slot) and a pixel of
-// its box.  Images up to 8192 px on a side pack (slot << 26 | dy << 13 | dx), so the exact test
-// needs no division; larger ones keep (slot << 24 | dy * bw + dx), where the depth pass's boxes
-// stay below 2^24 px (large_box <= 2^20) and the large-face pass, whose one face is slot 0,
-// gives all 32 bits to the pixel index.
-constexpr int kCandCap = 512;
+__device__ __forceinline__ void stage_face(FaceStage* dst, const TriSetup& t, float z0, float z1, float z2) {
+  dst->q[0] = make_float4(t.v0.x, t.v0.y, t.v0.z, t.v1.x);
+  dst->q[1] = make_float4(t.v1.y, t.v1.z, t.v2.x, t.v2.y);
+  dst->q[2] = make_float4(t.v2.z, t.N.x, t.N.y, t.N.z);
+  dst->q[3] = make_float4(t.dv0N, z0, z1, z2);
+}
+
+// Registers of the face a lane staged (its staging slot is that lane), read by the lanes that
+// test its candidates with a cross-lane permute.  The large-face pass holds its one face in
+// every lane.
+struct StageOwner {
+  int f, bc, x0, y0, bw;
+};
+
+// Candidate list of one wavefront: 32-bit words naming a staged face (its slot) and a pixel.
+// Images up to 8192 px on a side pack the pixel itself (slot << 26 | v << 13 | u); larger ones
+// keep (slot << 24 | dy * bw + dx) within the face's box, where the depth pass's boxes stay
+// below 2^24 px (large_box <= 2^20) and the large-face pass, whose one face is slot 0, gives
+// all 32 bits to the box index.
+constexpr int kCandCap = 256;        // depth pass
+constexpr int kCandCapLarge = 512;   // large-face pass (one staged face, LDS is not the limit)
 __device__ __forceinline__ bool cand_packed(int W, int H) { return W <= 8192 && H <= 8192; }
-__device__ __forceinline__ unsigned cand_word(bool packed, bool single, int slot, int dx, int dy, int bw) {
-  if (packed) return ((unsigned)slot << 26) | ((unsigned)dy << 13) | (unsigned)dx;
+__device__ __forceinline__ unsigned cand_word(bool packed, bool single, int slot, int u, int v, int dx, int dy,
+                                              int bw) {
+  if (packed) return ((unsigned)slot << 26) | ((unsigned)v << 13) | (unsigned)u;
   const unsigned p = (unsigned)dy * (unsigned)bw + (unsigned)dx;
   return single ? p : (((unsigned)slot << 24) | p);
 }
@@ -220,50 +239,66 @@ __device__ __forceinline__ unsigned cand_word(bool packed, bool single, int slot
 // Exact test of every staged candidate (reference uv2barycentric + inside test + depth key,
 // CUDABasedRasterization.cu:237-250), then the 64-bit minimum into the visibility buffer.
 // Each lane takes candidates lane, lane+64, ...; all lanes run the expensive test together.
-// `single`: the candidates of the large-face pass, which stages one face (slot 0).
-__device__ __forceinline__ void flush_candidates(const FaceStage* st, const unsigned* cand, int count,
-                                                 const Camera* __restrict__ cams, bool one_view, int bc0,
+// `single`: the candidates of the large-face pass, which stages one face (slot 0).  Called in
+// wave-uniform control flow: the owner reads need every lane active.
+__device__ __forceinline__ void flush_candidates(const FaceStage* st, const StageOwner& own, const unsigned* cand,
+                                                 int count, const Camera* __restrict__ cams, bool one_view, int bc0,
                                                  unsigned long long* __restrict__ keys, int W, int H, bool single,
                                                  TileMarks marks = TileMarks{}) {
   const int lane = threadIdx.x & 63;
   const bool packed = cand_packed(W, H);
-  for (int i = lane; i < count; i += 64) {
-    const unsigned c = cand[i];
-    int slot, dx, dy;
+  for (int base = 0; base < count; base += 64) {
+    const int i = base + lane;
+    const bool valid = i < count;
+    const unsigned c = valid ? cand[i] : 0u;
+    int slot, u, v;
     if (packed) {
       slot = single ? 0 : (int)(c >> 26);
-      dy = (int)((c >> 13) & 0x1fffu);
-      dx = (int)(c & 0x1fffu);
+      v = (int)((c >> 13) & 0x1fffu);
+      u = (int)(c & 0x1fffu);
     } else {
       slot = single ? 0 : (int)(c >> 24);
-      const unsigned p = single ? c : (c & 0xffffffu), w = (unsigned)st[slot].bw;
-      dy = (int)(p / w);
-      dx = (int)(p - (unsigned)dy * w);
+      const unsigned p = single ? c : (c & 0xffffffu);
+      const int x0 = __shfl(own.x0, slot, 64), y0 = __shfl(own.y0, slot, 64);
+      const unsigned w = (unsigned)__shfl(own.bw, slot, 64);
+      const unsigned dy = valid ? p / w : 0u;
+      u = x0 + (int)(p - dy * w);
+      v = y0 + (int)dy;
     }
-    const FaceStage& fs = st[slot];
-    const int u = fs.x0 + dx, v = fs.y0 + dy;
-    const Camera& cam = cams[one_view ? bc0 : fs.bc];
+    const int f = __shfl(own.f, slot, 64);
+    const int bc = one_view ? bc0 : __shfl(own.bc, slot, 64);
+    if (!valid) continue;
+    const float4 q0 = st[slot].q[0], q1 = st[slot].q[1], q2 = st[slot].q[2], q3 = st[slot].q[3];
+    const Camera& cam = cams[bc];
     const V3 d = pixel_ray(cam, u + 0.5f, v + 0.5f);
-    const V3 abc = ray_barycentric(fs.t, cam_origin_scaled(cam), d);
+    TriSetup t;
+    t.v0 = v3(q0.x, q0.y, q0.z);
+    t.v1 = v3(q0.w, q1.x, q1.y);
+    t.v2 = v3(q1.z, q1.w, q2.x);
+    t.N = v3(q2.y, q2.z, q2.w);
+    t.dv0N = q3.x;
+    t.den = dot(t.N, t.N);      // as tri_setup forms it
+    const V3 abc = ray_barycentric(t, cam_origin_scaled(cam), d);
     if (!inside(abc)) continue;
-    const uint64_t k = pack_key(depth_key(abc, fs.z0, fs.z1, fs.z2), fs.f);
-    atomicMin(keys + ((size_t)fs.bc * H + v) * W + u, (unsigned long long)k);
+    const uint64_t k = pack_key(depth_key(abc, q3.y, q3.z, q3.w), f);
+    atomicMin(keys + ((size_t)bc * H + v) * W + u, (unsigned long long)k);
     // screen-tile depth pass: the tile kernel merges this tile's visibility keys
-    if (marks.dirty) marks.dirty[((size_t)fs.bc * marks.tiles_y + (v >> marks.sy)) * marks.tiles_x + (u >> marks.sx)] = 1u;
+    if (marks.dirty) marks.dirty[((size_t)bc * marks.tiles_y + (v >> marks.sy)) * marks.tiles_x + (u >> marks.sx)] = 1u;
   }
 }
 
 // Pre-test the box of staged face `slot` lane-parallel (64 pixels per round) and append the
 // passing pixels to the candidate list; flushes first whenever a round could overflow it.
 __device__ __forceinline__ void collect_face(const EdgeTest& et_in, int slot, int x0, int y0, int bw, int bh,
-                                             FaceStage* st, unsigned* cand, int& count, const Camera* __restrict__ cams,
-                                             bool one_view, int bc0, unsigned long long* __restrict__ keys, int W, int H) {
+                                             FaceStage* st, const StageOwner& own, unsigned* cand, int& count,
+                                             const Camera* __restrict__ cams, bool one_view, int bc0,
+                                             unsigned long long* __restrict__ keys, int W, int H) {
   const int lane = threadIdx.x & 63;
   const int area = bw * bh;
   for (int base = 0; base < area; base += 64) {
     if (count > kCandCap - 64) {
       __syncthreads();
-      flush_candidates(st, cand, count, cams, one_view, bc0, keys, W, H, false);
+      flush_candidates(st, own, cand, count, cams, one_view, bc0, keys, W, H, false);
       __syncthreads();
       count = 0;
     }
@@ -278,7 +313,7 @@ __device__ __forceinline__ void collect_face(const EdgeTest& et_in, int slot, in
     const uint64_t m = __ballot(pass);
     if (pass) {
       const int off = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
-      cand[count + off] = cand_word(cand_packed(W, H), false, slot, dx, dy, bw);
+      cand[count + off] = cand_word(cand_packed(W, H), false, slot, x0 + dx, y0 + dy, dx, dy, bw);
     }
     count += __popcll(m);
   }
@@ -335,11 +370,7 @@ __device__ __forceinline__ void raster_body(const RasterArgs& ra, int64_t wg) {
         to_queue = true;
       } else {
         small = true;
-        FaceStage s;
-        s.t = tris[(size_t)(bc / C) * F + f];
-        s.z0 = r.p0.z; s.z1 = r.p1.z; s.z2 = r.p2.z;
-        s.f = f; s.x0 = r.bx.x0; s.y0 = r.bx.y0; s.bw = bw; s.bc = bc;
-        st[lane] = s;
+        stage_face(&st[lane], tris[(size_t)(bc / C) * F + f], r.p0.z, r.p1.z, r.p2.z);
         et = edge_test(r.p0, r.p1, r.p2);
         fx0 = r.bx.x0;
         fy0 = r.bx.y0;
@@ -362,6 +393,7 @@ __device__ __forceinline__ void raster_body(const RasterArgs& ra, int64_t wg) {
   if (!__ballot(small)) return;
   const int bc0 = __builtin_amdgcn_readfirstlane(bc);
   const bool one_view = __ballot(bc != bc0 && in_range) == 0;
+  const StageOwner own{f, bc, fx0, fy0, bw};
   __syncthreads();
   int count = 0;
   // tiny boxes: each lane walks its own box; pixel p of every lane's face per iteration
@@ -370,12 +402,12 @@ __device__ __forceinline__ void raster_body(const RasterArgs& ra, int64_t wg) {
   for (int p = 0; __ballot(p < my_area); ++p) {
     if (count > kCandCap - 64) {
       __syncthreads();
-      flush_candidates(st, cand, count, cams, one_view, bc0, keys, W, H, false);
+      flush_candidates(st, own, cand, count, cams, one_view, bc0, keys, W, H, false);
       __syncthreads();
       count = 0;
     }
     const bool pass = p < my_area && et.pass(fx0 + dx + 0.5f, fy0 + dy + 0.5f);
-    const unsigned word = cand_word(cand_packed(W, H), false, lane, dx, dy, bw);
+    const unsigned word = cand_word(cand_packed(W, H), false, lane, fx0 + dx, fy0 + dy, dx, dy, bw);
     if (++dx == bw) { dx = 0; ++dy; }
     const uint64_t m = __ballot(pass);
     if (pass) {
@@ -392,10 +424,10 @@ __device__ __forceinline__ void raster_body(const RasterArgs& ra, int64_t wg) {
     const EdgeTest ej = readlane_edges(et, j);
     const int x0 = __builtin_amdgcn_readlane(fx0, j), y0 = __builtin_amdgcn_readlane(fy0, j);
     const int jbw = __builtin_amdgcn_readlane(bw, j), jbh = __builtin_amdgcn_readlane(bh, j);
-    collect_face(ej, j, x0, y0, jbw, jbh, st, cand, count, cams, one_view, bc0, keys, W, H);
+    collect_face(ej, j, x0, y0, jbw, jbh, st, own, cand, count, cams, one_view, bc0, keys, W, H);
   }
   __syncthreads();
-  flush_candidates(st, cand, count, cams, one_view, bc0, keys, W, H, false);
+  flush_candidates(st, own, cand, count, cams, one_view, bc0, keys, W, H, false);
 }
 
 __global__ void __launch_bounds__(64) k_raster(RasterArgs ra) { raster_body(ra, xcd_contiguous(blockIdx.x, gridDim.x)); }
@@ -447,13 +479,8 @@ __device__ __forceinline__ void raster_large_face(const TriSetup* __restrict__ t
   const RasterFace r = load_face(faces, proj + (size_t)bc * N, f, W, H);
   const int bw = r.bx.x1 - r.bx.x0 + 1, bh = r.bx.y1 - r.bx.y0 + 1;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    FaceStage s;
-    s.t = tris[(size_t)(bc / C) * F + f];
-    s.z0 = r.p0.z; s.z1 = r.p1.z; s.z2 = r.p2.z;
-    s.f = f; s.x0 = r.bx.x0; s.y0 = r.bx.y0; s.bw = bw; s.bc = bc;
-    st[0] = s;
-  }
+  if (threadIdx.x == 0) stage_face(&st[0], tris[(size_t)(bc / C) * F + f], r.p0.z, r.p1.z, r.p2.z);
+  const StageOwner own{f, bc, r.bx.x0, r.bx.y0, bw};
   const EdgeTest et = edge_test(r.p0, r.p1, r.p2);
   __syncthreads();
   int count = 0;
@@ -477,9 +504,9 @@ __device__ __forceinline__ void raster_large_face(const TriSetup* __restrict__ t
     s_start[lane] = incl - len;
     __syncthreads();
     for (int q0 = 0; q0 < total; q0 += 64) {
-      if (count > kCandCap - 64) {
+      if (count > kCandCapLarge - 64) {
         __syncthreads();
-        flush_candidates(st, cand, count, cams, true, bc, keys, W, H, true, marks);
+        flush_candidates(st, own, cand, count, cams, true, bc, keys, W, H, true, marks);
         __syncthreads();
         count = 0;
       }
@@ -495,7 +522,7 @@ __device__ __forceinline__ void raster_large_face(const TriSetup* __restrict__ t
         const int x = s_xl[row] + (q - s_start[row]);
         const int yy = r.bx.y0 + rbase + row;
         pass = et.pass(x + 0.5f, yy + 0.5f);
-        p = (int)cand_word(cand_packed(W, H), true, 0, x - r.bx.x0, yy - r.bx.y0, bw);
+        p = (int)cand_word(cand_packed(W, H), true, 0, x, yy, x - r.bx.x0, yy - r.bx.y0, bw);
       }
       const uint64_t m = __ballot(pass);
       if (pass) {
@@ -506,7 +533,7 @@ __device__ __forceinline__ void raster_large_face(const TriSetup* __restrict__ t
     }
   }
   __syncthreads();
-  flush_candidates(st, cand, count, cams, true, bc, keys, W, H, true, marks);
+  flush_candidates(st, own, cand, count, cams, true, bc, keys, W, H, true, marks);
 }
 
 // Persistent wavefronts drain the large-face queue: wavefront w rasterises faces w, w + G,
@@ -520,7 +547,7 @@ __global__ void __launch_bounds__(64) k_raster_large(const TriSetup* __restrict_
                                                      unsigned int* __restrict__ counters, const int4* __restrict__ large,
                                                      int C, int N, int F, int W, int H, TileMarks marks) {
   __shared__ FaceStage st[1];
-  __shared__ unsigned cand[kCandCap];
+  __shared__ unsigned cand[kCandCapLarge];
   __shared__ int s_xl[64], s_start[64];
   const unsigned int n_large = counters[0];
   for (unsigned int item = blockIdx.x; item < n_large; item += gridDim.x)
@@ -534,7 +561,7 @@ __global__ void __launch_bounds__(64) k_raster_large_multi(const RasterMesh* __r
                                                            unsigned int* __restrict__ counters,
                                                            const int4* __restrict__ large) {
   __shared__ FaceStage st[1];
-  __shared__ unsigned cand[kCandCap];
+  __shared__ unsigned cand[kCandCapLarge];
   __shared__ int s_xl[64], s_start[64];
   const unsigned int n_large = counters[0];
   for (unsigned int item = blockIdx.x; item < n_large; item += gridDim.x) {
