@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libgvv_amd.so")
 SOURCES = ["gvv_forward.hip", "gvv_backward.hip", "gvv_capi.hip", "gvv_conv.hip", "gvv_multi.hip", "gvv_calib.hip",
-           "gvv_noalloc.hip", "gvv_tile.hip"]
+           "gvv_noalloc.hip", "gvv_tile.hip", "gvv_fit.hip"]
 ARCH = os.environ.get("GVV_OFFLOAD_ARCH", "gfx950")
 
 FLAGS = [

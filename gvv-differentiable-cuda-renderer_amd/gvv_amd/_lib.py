@@ -1,5 +1,5 @@
 """ctypes binding of libgvv_amd.so (C ABI declared in include/gvv_renderer.h,
-include/gvv_multi.h and include/gvv_proposal.h).
+include/gvv_multi.h, include/gvv_proposal.h and include/gvv_fit.h).
 
 The library is the only compute path: if it has not been built this module raises; there is
 no CPU fallback.
@@ -21,7 +21,8 @@ EXPORTS = ("gvv_last_error", "gvv_version", "gvv_create", "gvv_destroy", "gvv_fo
            "gvv_conv_backward_weight", "gvv_conv_prepare_weights", "gvv_pack_mosaic",
            "gvv_multi_create", "gvv_multi_destroy", "gvv_multi_run", "gvv_multi_use_graph", "gvv_multi_autotune",
            "gvv_multi_set_batched", "gvv_multi_set_tuning",
-           "gvv_multi_tuning")
+           "gvv_multi_tuning",
+           "gvv_masked_sq_workspace_floats", "gvv_masked_sq_error", "gvv_masked_sq_error_grad")
 
 _lib = None
 
@@ -96,6 +97,12 @@ def load():
     L.gvv_multi_autotune.restype = ctypes.c_int
     L.gvv_multi_tuning.argtypes = [P, i32, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32)]
     L.gvv_multi_tuning.restype = ctypes.c_int
+    L.gvv_masked_sq_workspace_floats.argtypes = [i32, i64]
+    L.gvv_masked_sq_workspace_floats.restype = i64
+    L.gvv_masked_sq_error.argtypes = [i32, i64, P, P, P, P, P, P]
+    L.gvv_masked_sq_error.restype = ctypes.c_int
+    L.gvv_masked_sq_error_grad.argtypes = [i32, i64, P, P, P, P, P, P]
+    L.gvv_masked_sq_error_grad.restype = ctypes.c_int
     _lib = L
     return L
 

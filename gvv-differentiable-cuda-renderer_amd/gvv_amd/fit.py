@@ -15,7 +15,9 @@ workload configuration it is measured on:
 * the template's vertex colours are U(0.2, 0.9) per vertex (a high-frequency albedo);
   FitConfig.colour_smoothing > 0 smooths them over the mesh for a low-frequency albedo;
 * vertexColor + shaded rendering of 4 views at 512 x 512, masked L2 photometric loss against
-  targets rendered from ground-truth parameters, Adam (lr 1e-2).
+  targets rendered from ground-truth parameters, Adam (lr 1e-2).  The loss and its gradient
+  with respect to the render run on the library's own kernels (include/gvv_fit.h): one
+  deterministic pass each over render, target and mask (MaskedSquaredError).
 
 Lengths are millimetres inside the renderer (the reference's thresholds assume mm).
 """
@@ -28,8 +30,8 @@ from dataclasses import dataclass, field
 import numpy as np
 import torch
 
-from . import scenes
-from .renderer import Rasterizer, RenderFunction
+from . import _lib, scenes
+from .renderer import Rasterizer, RenderFunction, _ptr, _stream_ptr
 
 MM = 1000.0
 
@@ -191,6 +193,50 @@ def plain_subject_losses(model: "ShapeModel", target, mask):
         return (d * d).flatten(1).sum(1) / mask.flatten(1).sum(1).clamp_min(1.0)
 
 
+class MaskedSquaredError(torch.autograd.Function):
+    """Per-subject masked squared error sq[b] = sum over pixels and channels of
+    ((render - target) * mask)^2, for render/target [B, ..., 3] and mask [B, ..., 1] (the same
+    pixels), on gvv_masked_sq_error; the backward (gvv_masked_sq_error_grad) gives the render
+    gradient 2 (render - target) mask^2 g[b].  Target and mask take no gradient."""
+
+    @staticmethod
+    def forward(ctx, render, target, mask):
+        B = render.shape[0]
+        if target.shape != render.shape or render.shape[-1] != 3 or mask.shape != render.shape[:-1] + (1,):
+            raise ValueError(f"masked squared error: render {tuple(render.shape)}, target {tuple(target.shape)} and "
+                             f"mask {tuple(mask.shape)} must be [B, ..., 3], [B, ..., 3] and [B, ..., 1]")
+        for name, t in (("render", render), ("target", target), ("mask", mask)):
+            if t.dtype != torch.float32 or not t.is_cuda:
+                raise ValueError(f"masked squared error: {name} must be a float32 device tensor")
+        render, target, mask = render.contiguous(), target.contiguous(), mask.contiguous()
+        P = mask.numel() // B
+        L = _lib.load()
+        dev = render.device
+        work = torch.empty(int(L.gvv_masked_sq_workspace_floats(B, P)), device=dev)
+        sq = torch.empty(B, device=dev)
+        _lib.check(L.gvv_masked_sq_error(B, P, _ptr(render), _ptr(target), _ptr(mask), _ptr(work), _ptr(sq),
+                                         _stream_ptr(dev)), "gvv_masked_sq_error")
+        ctx.save_for_backward(render, target, mask)
+        return sq
+
+    @staticmethod
+    def backward(ctx, g_sq):
+        render, target, mask = ctx.saved_tensors
+        B = render.shape[0]
+        P = mask.numel() // B
+        g_sq = g_sq.contiguous()
+        g_render = torch.empty_like(render)
+        L = _lib.load()
+        _lib.check(L.gvv_masked_sq_error_grad(B, P, _ptr(render), _ptr(target), _ptr(mask), _ptr(g_sq),
+                                              _ptr(g_render), _stream_ptr(render.device)), "gvv_masked_sq_error_grad")
+        return g_render, None, None
+
+
+def masked_sq_error(render, target, mask):
+    """Per-subject masked squared error [B] (MaskedSquaredError)."""
+    return MaskedSquaredError.apply(render, target, mask)
+
+
 def masked_l2(render, target, mask):
     """Mean squared photometric error over the target's foreground pixels, per subject summed."""
     d = (render - target) * mask
@@ -238,8 +284,7 @@ def fit(model: ShapeModel, target, mask, init: Params, iters=None, lr=None, stop
     for it in range(iters):
         opt.zero_grad(set_to_none=True)
         render, _ = model.render(Params.from_flat(x, cfg.n_basis))
-        d = (render - target) * mask
-        sq = (d * d).flatten(1).sum(1)                       # per subject
+        sq = masked_sq_error(render, target, mask)           # per subject
         loss = sq.sum() / mask_sub.sum().clamp_min(1.0)
         back = None
         if it == 0 and fallback is not None:

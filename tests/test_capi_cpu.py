@@ -1,5 +1,5 @@
 """The C ABI without a GPU: libgvv_amd.so loads, exports every function include/gvv_renderer.h,
-include/gvv_multi.h and include/gvv_proposal.h declare, and rejects invalid arguments on the host before any
+include/gvv_multi.h, include/gvv_proposal.h and include/gvv_fit.h declare, and rejects invalid arguments on the host before any
 device call."""
 import ctypes
 import os
@@ -12,7 +12,7 @@ from gvv_amd import _lib
 from gvv_amd.renderer import Rasterizer
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADERS = [os.path.join(ROOT, "include", h) for h in ("gvv_renderer.h", "gvv_multi.h", "gvv_proposal.h")]
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("gvv_renderer.h", "gvv_multi.h", "gvv_proposal.h", "gvv_fit.h")]
 
 
 def declared_functions():
@@ -125,3 +125,16 @@ def test_renderer_cache_key_separates_streams_and_devices():
     assert k != cache_key(*args, 1, 1234)
     assert k != cache_key(faces, tc, 3, 1, 8, 8, "textured", "shaded", 2, 2, False, 0, 1234)
     assert k != cache_key([0, 2, 1], tc, 3, 1, 8, 8, "vertexColor", "shaded", 2, 2, False, 0, 1234)
+
+
+def test_masked_sq_error_rejects_invalid_sizes_on_the_host():
+    L = _lib.load()
+    assert L.gvv_masked_sq_workspace_floats(0, 10) == -1
+    assert L.gvv_masked_sq_workspace_floats(2, 0) == -1
+    assert L.gvv_masked_sq_workspace_floats(2, 4096) == 2
+    assert L.gvv_masked_sq_workspace_floats(3, 4097) == 6
+    assert L.gvv_masked_sq_error(0, 10, None, None, None, None, None, None) != 0
+    assert "subjects" in L.gvv_last_error().decode()
+    assert L.gvv_masked_sq_error(1, 10, None, None, None, None, None, None) != 0
+    assert "null" in L.gvv_last_error().decode()
+    assert L.gvv_masked_sq_error_grad(1, 0, None, None, None, None, None, None) != 0

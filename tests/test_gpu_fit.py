@@ -127,3 +127,45 @@ def test_short_plain_fit_reduces_loss(device):
     res = fit.fit(model, target, mask, fit.plain_init(cfg.batch, cfg.n_basis, device), iters=60)
     print("plain fit loss", res.losses[0], "->", res.losses[-1], flush=True)
     assert res.losses[-1] < 0.7 * res.losses[0]
+
+
+@pytest.mark.parametrize("B,shape", [(1, (1, 1, 1)), (3, (2, 37, 53)), (16, (4, 64, 64)), (2, (1, 300, 301))])
+def test_masked_squared_error_kernel(device, B, shape):
+    """The fit's loss kernel (gvv_masked_sq_error and its gradient) against the same expression
+    in float64 torch: per-subject sums within 1e-5 relative, the render gradient within 1e-6
+    of the largest entry; ragged pixel counts (a partial last workgroup) and one pixel."""
+    g = torch.Generator(device="cpu").manual_seed(sum(shape) + B)
+    r = torch.rand((B,) + shape + (3,), generator=g).to(device)
+    t = torch.rand((B,) + shape + (3,), generator=g).to(device)
+    m = (torch.rand((B,) + shape + (1,), generator=g) > 0.3).float().to(device)
+    m[0].view(-1)[0] = 1.0
+    rr = r.clone().requires_grad_(True)
+    sq = fit.masked_sq_error(rr, t, m)
+    w = torch.rand(B, generator=g).to(device)
+    (sq * w).sum().backward()
+    d = (r.double() - t.double()) * m.double()
+    ref = (d * d).flatten(1).sum(1)
+    err = ((sq.double() - ref).abs() / ref.abs().clamp_min(1e-30)).max().item()
+    gref = 2.0 * d * m.double() * w.double().view((B,) + (1,) * (r.dim() - 1))
+    gerr = ((rr.grad.double() - gref).abs().max() / gref.abs().max().clamp_min(1e-30)).item()
+    print(f"masked sq error B={B} {shape}: sum rel err {err:.2e}, grad rel err {gerr:.2e}", flush=True)
+    assert err <= 1e-5 and gerr <= 1e-6
+
+
+def test_masked_squared_error_is_deterministic(device):
+    """Fixed-order sums: two evaluations of the same inputs agree bit for bit."""
+    g = torch.Generator(device="cpu").manual_seed(11)
+    r = torch.rand((4, 3, 128, 128, 3), generator=g).to(device)
+    t = torch.rand((4, 3, 128, 128, 3), generator=g).to(device)
+    m = (torch.rand((4, 3, 128, 128, 1), generator=g) > 0.5).float().to(device)
+    a = fit.masked_sq_error(r, t, m)
+    b = fit.masked_sq_error(r, t, m)
+    assert torch.equal(a, b)
+
+
+def test_masked_squared_error_rejects_mismatched_shapes(device):
+    r = torch.zeros((2, 4, 4, 3), device=device)
+    with pytest.raises(ValueError):
+        fit.masked_sq_error(r, torch.zeros((2, 4, 5, 3), device=device), torch.zeros((2, 4, 4, 1), device=device))
+    with pytest.raises(ValueError):
+        fit.masked_sq_error(r, r, torch.zeros((2, 4, 4, 3), device=device))
