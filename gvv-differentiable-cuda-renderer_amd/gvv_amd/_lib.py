@@ -22,7 +22,8 @@ EXPORTS = ("gvv_last_error", "gvv_version", "gvv_create", "gvv_destroy", "gvv_fo
            "gvv_multi_create", "gvv_multi_destroy", "gvv_multi_run", "gvv_multi_use_graph", "gvv_multi_autotune",
            "gvv_multi_set_batched", "gvv_multi_set_tuning",
            "gvv_multi_tuning",
-           "gvv_masked_sq_workspace_floats", "gvv_masked_sq_error", "gvv_masked_sq_error_grad")
+           "gvv_masked_sq_workspace_floats", "gvv_masked_sq_error", "gvv_masked_sq_error_grad",
+           "gvv_shape_forward", "gvv_shape_workspace_floats", "gvv_shape_backward")
 
 _lib = None
 
@@ -103,6 +104,13 @@ def load():
     L.gvv_masked_sq_error.restype = ctypes.c_int
     L.gvv_masked_sq_error_grad.argtypes = [i32, i64, P, P, P, P, P, P]
     L.gvv_masked_sq_error_grad.restype = ctypes.c_int
+    f32 = ctypes.c_float
+    L.gvv_shape_forward.argtypes = [i32, i32, i32, P, P, f32] + [P] * 7
+    L.gvv_shape_forward.restype = ctypes.c_int
+    L.gvv_shape_workspace_floats.argtypes = [i32, i32, i32]
+    L.gvv_shape_workspace_floats.restype = i64
+    L.gvv_shape_backward.argtypes = [i32, i32, i32, P, P, P, P, f32, P, P, P, P]
+    L.gvv_shape_backward.restype = ctypes.c_int
     _lib = L
     return L
 

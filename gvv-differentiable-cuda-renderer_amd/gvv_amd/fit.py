@@ -137,11 +137,23 @@ class ShapeModel:
         self._target_dummy = {}
 
     # -- geometry ------------------------------------------------------------------------------
-    def vertices(self, p: Params) -> torch.Tensor:
-        B = p.alpha.shape[0]
-        X = self.template + (p.alpha @ self.basis).view(B, self.N, 3)
+    def motion(self, p: Params) -> torch.Tensor:
+        """Per-subject rigid motion [B, 12]: rotation (row-major) and translation in mm."""
+        B = p.pose.shape[0]
         R = rodrigues(p.pose[:, :3])
         t = p.pose[:, 3:6] * (0.1 * MM)
+        return torch.cat([R.reshape(B, 9), t], 1)
+
+    def vertices(self, p: Params) -> torch.Tensor:
+        """Posed vertices [B, N, 3] on the shape-model kernels (ShapeVertices)."""
+        return ShapeVertices.apply(p.alpha, p.pose, self.template, self.basis, self.centre)
+
+    def vertices_reference(self, p: Params) -> torch.Tensor:
+        """The same in torch operations (the kernels' reference)."""
+        B = p.alpha.shape[0]
+        X = self.template + (p.alpha @ self.basis).view(B, self.N, 3)
+        m = self.motion(p)
+        R, t = m[:, :9].view(B, 3, 3), m[:, 9:]
         return torch.matmul(X - self.centre, R.transpose(1, 2).contiguous()) + self.centre + t[:, None, :]
 
     def render(self, p: Params):
@@ -191,6 +203,52 @@ def plain_subject_losses(model: "ShapeModel", target, mask):
         r = model.render(plain_init(1, model.cfg.n_basis, model.device))[0]
         d = (r - target) * mask
         return (d * d).flatten(1).sum(1) / mask.flatten(1).sum(1).clamp_min(1.0)
+
+
+class ShapeVertices(torch.autograd.Function):
+    """v[b] = R_b (template + alpha[b] @ basis - centre) + centre + t_b on gvv_shape_forward, with
+    R_b = rodrigues(pose[b, :3]) and t_b = pose[b, 3:6] x 0.1 m, and the gradients of alpha and
+    pose from gvv_shape_backward.  Template [N, 3], basis [K, 3N] and centre [3] take no
+    gradient."""
+
+    TSCALE = 0.1 * MM     # translation units of the pose, in mm
+
+    @staticmethod
+    def forward(ctx, alpha, pose, template, basis, centre):
+        B, K = alpha.shape
+        N = template.shape[0]
+        if pose.shape != (B, 6) or template.shape != (N, 3) or basis.shape != (K, 3 * N) or centre.numel() != 3:
+            raise ValueError(f"shape vertices: alpha {tuple(alpha.shape)}, pose {tuple(pose.shape)}, template "
+                             f"{tuple(template.shape)}, basis {tuple(basis.shape)}, centre {tuple(centre.shape)}")
+        for name, t in (("alpha", alpha), ("pose", pose), ("template", template), ("basis", basis),
+                        ("centre", centre)):
+            if t.dtype != torch.float32 or not t.is_cuda:
+                raise ValueError(f"shape vertices: {name} must be a float32 device tensor")
+        alpha, pose, template, basis, centre = (t.contiguous() for t in (alpha, pose, template, basis, centre))
+        dev = alpha.device
+        v = torch.empty((B, N, 3), device=dev)
+        xc = torch.empty((B, N, 3), device=dev)
+        rt = torch.empty((B, 12), device=dev)
+        L = _lib.load()
+        _lib.check(L.gvv_shape_forward(B, N, K, _ptr(alpha), _ptr(pose), ShapeVertices.TSCALE, _ptr(template),
+                                       _ptr(basis), _ptr(centre), _ptr(v), _ptr(xc), _ptr(rt), _stream_ptr(dev)),
+                   "gvv_shape_forward")
+        ctx.save_for_backward(xc, pose, rt, basis)
+        return v
+
+    @staticmethod
+    def backward(ctx, g_v):
+        xc, pose, rt, basis = ctx.saved_tensors
+        B, N, _ = xc.shape
+        K = basis.shape[0]
+        g_v = g_v.contiguous()
+        L = _lib.load()
+        work = torch.empty(int(L.gvv_shape_workspace_floats(B, N, K)), device=xc.device)
+        g = torch.empty((B, K + 6), device=xc.device)
+        _lib.check(L.gvv_shape_backward(B, N, K, _ptr(g_v), _ptr(xc), _ptr(pose), _ptr(rt), ShapeVertices.TSCALE,
+                                        _ptr(basis), _ptr(work), _ptr(g), _stream_ptr(xc.device)),
+                   "gvv_shape_backward")
+        return g[:, :K], g[:, K:], None, None, None
 
 
 class MaskedSquaredError(torch.autograd.Function):
@@ -265,14 +323,19 @@ class FitResult:
 def fit(model: ShapeModel, target, mask, init: Params, iters=None, lr=None, stop_at=None, t0=None,
         fallback=None) -> FitResult:
     """Adam over (shape, SH, pose).  `losses[i]` is the loss of the parameters before step i+1;
-    `times[i]` the wall-clock since t0 (default: entry) once that loss is known.  Stops as soon
-    as the loss is <= stop_at (before that iteration's backward).
+    `times[i]` the wall-clock since t0 (default: entry) once that loss is known on the host.
+    Stops as soon as a loss is <= stop_at, returning the parameters that reached it.
+
+    The host reads each iteration's loss one iteration late: iteration i+1's forward is queued
+    before the host waits for loss i (an event after its copy to pinned memory), so the GPU does
+    not idle while the host queues work.  Stopping at loss i therefore drops the forward already
+    queued for iteration i+1 and returns the parameters of iteration i (kept on the device).
 
     `fallback` = (plain parameters, their per-subject losses [B]): the first iteration's forward
-    also yields each subject's own loss at `init`; a subject whose loss there is worse than its
-    plain loss falls back to the plain parameters (its first update is skipped, as its gradient
-    was taken at the discarded start) and the recorded first loss is that of the chosen starts.
-    `FitResult.fallbacks` counts them."""
+    also yields each subject's own loss at `init` (read at once, not late); a subject whose loss
+    there is worse than its plain loss falls back to the plain parameters (its first update is
+    skipped, as its gradient was taken at the discarded start) and the recorded first loss is
+    that of the chosen starts.  `FitResult.fallbacks` counts them."""
     cfg = model.cfg
     iters = cfg.iters if iters is None else iters
     lr = cfg.lr if lr is None else lr
@@ -281,11 +344,27 @@ def fit(model: ShapeModel, target, mask, init: Params, iters=None, lr=None, stop
     res = FitResult(params=None)
     t0 = time.perf_counter() if t0 is None else t0
     mask_sub = mask.flatten(1).sum(1)
+    msum = mask_sub.sum().clamp_min(1.0)
+    host = torch.empty(2, dtype=torch.float32, pin_memory=True)     # in-flight losses, by iteration parity
+    snaps = [torch.empty_like(x.detach()) for _ in range(2)]          # the parameters they belong to
+    pending = None                                                    # (slot, event) of the unread loss
+    done = None                                                       # parameters that reached stop_at
+
+    def read(slot, ev):
+        ev.synchronize()
+        lv = float(host[slot])
+        res.losses.append(lv)
+        res.times.append(time.perf_counter() - t0)
+        return stop_at is not None and lv <= stop_at
+
     for it in range(iters):
+        slot = it & 1
+        with torch.no_grad():
+            snaps[slot].copy_(x)
         opt.zero_grad(set_to_none=True)
         render, _ = model.render(Params.from_flat(x, cfg.n_basis))
         sq = masked_sq_error(render, target, mask)           # per subject
-        loss = sq.sum() / mask_sub.sum().clamp_min(1.0)
+        loss = sq.sum() / msum
         back = None
         if it == 0 and fallback is not None:
             plain, plain_losses = fallback
@@ -294,22 +373,34 @@ def fit(model: ShapeModel, target, mask, init: Params, iters=None, lr=None, stop
             res.fallbacks = int(back.sum())
             if res.fallbacks:
                 chosen = torch.where(back, plain_losses * mask_sub, sq.detach())
-                lv = float(chosen.sum() / mask_sub.sum().clamp_min(1.0))
+                lv = float(chosen.sum() / msum)
+                with torch.no_grad():
+                    snaps[slot][back] = plain.flat()[back]
             else:
                 lv = float(loss.detach())
+            res.losses.append(lv)
+            res.times.append(time.perf_counter() - t0)
+            if stop_at is not None and lv <= stop_at:
+                done = snaps[slot]
+                break
         else:
-            lv = float(loss.detach())
-        res.losses.append(lv)
-        res.times.append(time.perf_counter() - t0)
-        if stop_at is not None and lv <= stop_at:
-            break
+            host[slot].copy_(loss.detach(), non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record()
+            if pending is not None and read(*pending):
+                done = snaps[slot ^ 1]
+                pending = None
+                break
+            pending = (slot, ev)
         loss.backward()
         if back is not None and res.fallbacks:
             with torch.no_grad():
                 x.grad[back] = 0.0
                 x[back] = plain.flat()[back]
         opt.step()
-    res.params = Params.from_flat(x.detach(), cfg.n_basis)
+    if pending is not None and read(*pending):
+        done = snaps[pending[0]]
+    res.params = Params.from_flat((x if done is None else done).detach().clone(), cfg.n_basis)
     return res
 
 

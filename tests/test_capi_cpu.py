@@ -138,3 +138,14 @@ def test_masked_sq_error_rejects_invalid_sizes_on_the_host():
     assert L.gvv_masked_sq_error(1, 10, None, None, None, None, None, None) != 0
     assert "null" in L.gvv_last_error().decode()
     assert L.gvv_masked_sq_error_grad(1, 0, None, None, None, None, None, None) != 0
+
+
+def test_shape_kernels_reject_invalid_sizes_on_the_host():
+    L = _lib.load()
+    assert L.gvv_shape_workspace_floats(2, 100, 0) == -1
+    assert L.gvv_shape_workspace_floats(2, 100, 129) == -1
+    assert L.gvv_shape_workspace_floats(2, 100, 8) == (2 + 1) * 2 * 20
+    assert L.gvv_shape_forward(1, 10, 200, None, None, 1.0, *([None] * 7)) != 0
+    assert "basis" in L.gvv_last_error().decode()
+    assert L.gvv_shape_backward(1, 10, 8, None, None, None, None, 1.0, None, None, None, None) != 0
+    assert "null" in L.gvv_last_error().decode()
