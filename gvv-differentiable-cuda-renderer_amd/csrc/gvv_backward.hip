@@ -122,8 +122,8 @@ GVV_HD V3 blend4(float su, float sv, float tu, float tv, V3 cLL, V3 cHL, V3 cLH,
 #pragma clang fp contract(off)
 
 // Per-pixel gradient terms of one covered pixel (reference Grad.cu:205-555).  Adds the SH
-// terms to sh_acc; returns the per-face terms (position, vertex-normal gradient and
-// vertex-colour rows for the three corners) in t and, textured, the texture-gradient term in
+// terms to sh_acc; returns the per-face terms (position, vertex-normal gradient and, when
+// G_COL >= 0, vertex-colour rows for the three corners) in t and, textured, the texture-gradient term in
 // tex_val for texel tex_key of the view's batch entry (left at -1 when there is none).
 // `cam` and `shc` are the view's camera and SH coefficients, held in registers by the caller.
 template <int ALBEDO, bool SHADED, bool M2D, bool HAVE_BARY, int G_POS, int G_VN, int G_COL>
@@ -194,9 +194,11 @@ __device__ __forceinline__ void pixel_terms(const BackwardArgs& a, const Camera&
   V3 alb = v3(0.f, 0.f, 0.f);
   if (ALBEDO == kVertexColor) {
     alb = c0 * abc.x + c1 * abc.y + c2 * abc.z;
-    const V3 ga = v3(g.x * L.x, g.y * L.y, g.z * L.z);
+    if (G_COL >= 0) {
+      const V3 ga = v3(g.x * L.x, g.y * L.y, g.z * L.z);
 #pragma unroll
-    for (int r = 0; r < 3; ++r) { t[G_COL + 3 * r] = ga.x * w3[r]; t[G_COL + 3 * r + 1] = ga.y * w3[r]; t[G_COL + 3 * r + 2] = ga.z * w3[r]; }
+      for (int r = 0; r < 3; ++r) { t[G_COL + 3 * r] = ga.x * w3[r]; t[G_COL + 3 * r + 1] = ga.y * w3[r]; t[G_COL + 3 * r + 2] = ga.z * w3[r]; }
+    }
   } else if (ALBEDO == kTextured) {
     const float LU = tc.LU + 0.5f, HU = tc.LU + 1.5f, LV = tc.LV + 0.5f, HV = tc.LV + 1.5f;
     const float U0 = tc.U0, V0 = tc.V0;
@@ -304,12 +306,13 @@ __device__ __forceinline__ void pixel_terms(const BackwardArgs& a, const Camera&
 // partial slots.
 constexpr int kFaceSlots = 96;
 
-template <int ALBEDO, bool SHADED, bool M2D, bool HAVE_BARY, int TILE>
+// COL_GRAD: the vertex-colour gradient is formed (vertexColor mode with a g_col output).
+template <int ALBEDO, bool SHADED, bool M2D, bool HAVE_BARY, int TILE, bool COL_GRAD>
 __device__ __forceinline__ void pixel_grad_body(const BackwardArgs& a, int64_t work) {
   static_assert(TILE == 8 || TILE == 16 || TILE == 32, "tile edge must be 8, 16 or 32");
   constexpr int RT = TILE / 8;                            // 8x8 rounds per tile edge
   constexpr bool NEED_POS = SHADED || M2D;
-  constexpr bool NEED_COL = ALBEDO == kVertexColor;
+  constexpr bool NEED_COL = ALBEDO == kVertexColor && COL_GRAD;
   constexpr int G_POS = 0, G_VN = NEED_POS ? 9 : 0, G_COL = G_VN + (SHADED ? 9 : 0);
   constexpr int NT = G_COL + (NEED_COL ? 9 : 0);         // per-face terms
   constexpr int NT1 = NT > 0 ? NT : 1;
@@ -404,7 +407,7 @@ __device__ __forceinline__ void pixel_grad_body(const BackwardArgs& a, int64_t w
     int tex_key = -1;
     float tex_val[3] = {0.f, 0.f, 0.f};
     if (idf >= 0)
-      pixel_terms<ALBEDO, SHADED, M2D, HAVE_BARY, G_POS, G_VN, G_COL>(a, cam, shv, bc, b, px, py, idx, idf, i0, i1, i2, t,
+      pixel_terms<ALBEDO, SHADED, M2D, HAVE_BARY, G_POS, G_VN, NEED_COL ? G_COL : -1>(a, cam, shv, bc, b, px, py, idx, idf, i0, i1, i2, t,
                                                                      sh_acc, tex_key, tex_val);
     if (round + 1 < RT * RT && nidf >= 0) {
       ni0 = a.faces[3 * nidf]; ni1 = a.faces[3 * nidf + 1]; ni2 = a.faces[3 * nidf + 2];
@@ -500,9 +503,9 @@ __device__ __forceinline__ void pixel_grad_body(const BackwardArgs& a, int64_t w
   }
 }
 
-template <int ALBEDO, bool SHADED, bool M2D, bool HAVE_BARY, int TILE>
+template <int ALBEDO, bool SHADED, bool M2D, bool HAVE_BARY, int TILE, bool COL_GRAD>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_pixel_grad(BackwardArgs a) {
-  pixel_grad_body<ALBEDO, SHADED, M2D, HAVE_BARY, TILE>(a, xcd_contiguous(blockIdx.x, gridDim.x));
+  pixel_grad_body<ALBEDO, SHADED, M2D, HAVE_BARY, TILE, COL_GRAD>(a, xcd_contiguous(blockIdx.x, gridDim.x));
 }
 
 // Fixed-order reduction of the SH tile partials ([b*C + c][27][tiles]): one workgroup per
@@ -611,21 +614,31 @@ FinishArgs finish_args(const BackwardArgs& a) {
   return f;
 }
 
-template <int ALBEDO, bool SHADED, bool M2D, bool HAVE_BARY>
+template <int ALBEDO, bool SHADED, bool M2D, bool HAVE_BARY, bool COL_GRAD>
 static void launch_pixel_grad_t(const BackwardArgs& a, hipStream_t s) {
   dim3 grid((unsigned)((int64_t)a.tiles_x * a.tiles_y * a.B * a.C));
-  if (a.tile == 8) hipLaunchKernelGGL((k_pixel_grad<ALBEDO, SHADED, M2D, HAVE_BARY, 8>), grid, dim3(64), 0, s, a);
-  else if (a.tile == 32) hipLaunchKernelGGL((k_pixel_grad<ALBEDO, SHADED, M2D, HAVE_BARY, 32>), grid, dim3(64), 0, s, a);
-  else hipLaunchKernelGGL((k_pixel_grad<ALBEDO, SHADED, M2D, HAVE_BARY, 16>), grid, dim3(64), 0, s, a);
+  if (a.tile == 8)
+    hipLaunchKernelGGL((k_pixel_grad<ALBEDO, SHADED, M2D, HAVE_BARY, 8, COL_GRAD>), grid, dim3(64), 0, s, a);
+  else if (a.tile == 32)
+    hipLaunchKernelGGL((k_pixel_grad<ALBEDO, SHADED, M2D, HAVE_BARY, 32, COL_GRAD>), grid, dim3(64), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_pixel_grad<ALBEDO, SHADED, M2D, HAVE_BARY, 16, COL_GRAD>), grid, dim3(64), 0, s, a);
 }
 
+template <int ALBEDO, bool SHADED, bool COL_GRAD>
+static void launch_pixel_grad_c(const BackwardArgs& a, hipStream_t s) {
+  const bool m2d = a.g_target != nullptr, hb = a.bary != nullptr;
+  if (m2d && hb) launch_pixel_grad_t<ALBEDO, SHADED, true, true, COL_GRAD>(a, s);
+  else if (m2d) launch_pixel_grad_t<ALBEDO, SHADED, true, false, COL_GRAD>(a, s);
+  else if (hb) launch_pixel_grad_t<ALBEDO, SHADED, false, true, COL_GRAD>(a, s);
+  else launch_pixel_grad_t<ALBEDO, SHADED, false, false, COL_GRAD>(a, s);
+}
+
+// The vertex-colour gradient's terms are formed only in vertexColor mode with a g_col output.
 template <int ALBEDO, bool SHADED>
 static void launch_pixel_grad_m(const BackwardArgs& a, hipStream_t s) {
-  const bool m2d = a.g_target != nullptr, hb = a.bary != nullptr;
-  if (m2d && hb) launch_pixel_grad_t<ALBEDO, SHADED, true, true>(a, s);
-  else if (m2d) launch_pixel_grad_t<ALBEDO, SHADED, true, false>(a, s);
-  else if (hb) launch_pixel_grad_t<ALBEDO, SHADED, false, true>(a, s);
-  else launch_pixel_grad_t<ALBEDO, SHADED, false, false>(a, s);
+  if (ALBEDO == kVertexColor && !a.g_col) launch_pixel_grad_c<ALBEDO, SHADED, false>(a, s);
+  else launch_pixel_grad_c<ALBEDO, SHADED, true>(a, s);
 }
 
 static inline unsigned blocks_for(int64_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }

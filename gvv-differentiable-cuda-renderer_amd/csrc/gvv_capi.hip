@@ -176,7 +176,7 @@ int plan_backward_into(Handle* h, int B, int texH, int texW, const float* g_rend
                        float* g_pv, float* sh_partial, FaceGrad* face_grad, BackwardPlan& p) {
   if (B <= 0) return fail("batch size must be > 0");
   if (texH <= 0 || texW <= 0) return fail("texture height/width must be > 0");
-  if (!g_pos || !g_col || !g_tex || !g_sh) return fail("null gradient output pointer");
+  if (!g_pos || !g_tex || !g_sh) return fail("null gradient output pointer");   // g_col may be NULL
   if (h->albedo == kNormal || h->albedo == kLighting) return fail("normal and lighting modes have no gradients");
   if (!g_render || !vpos || !vcol || !tex || !sh || !vnormal || !face || !extr || !intr)
     return fail("null tensor pointer");
@@ -325,13 +325,13 @@ int gvv_backward(void* handle, int32_t B, int32_t texH, int32_t texW, const floa
   if (check_device(h)) return 1;
   if (B <= 0) return fail("batch size must be > 0");
   if (texH <= 0 || texW <= 0) return fail("texture height/width must be > 0");
-  if (!g_pos || !g_col || !g_tex || !g_sh) return fail("null gradient output pointer");
+  if (!g_pos || !g_tex || !g_sh) return fail("null gradient output pointer");   // g_col may be NULL
   hipStream_t s = (hipStream_t)hip_stream;
   const int64_t nv = (int64_t)B * h->N * 3;
   if (h->albedo == kNormal || h->albedo == kLighting) {
     // the reference routes these modes to zero gradients (python/CudaRenderer.py:207-213)
     GVV_CHECK(launch_fill_f32(g_pos, nv, 0.f, s));
-    GVV_CHECK(launch_fill_f32(g_col, nv, 0.f, s));
+    if (g_col) GVV_CHECK(launch_fill_f32(g_col, nv, 0.f, s));
     GVV_CHECK(launch_fill_f32(g_tex, (int64_t)B * texH * texW * 3, 0.f, s));
     GVV_CHECK(launch_fill_f32(g_sh, (int64_t)B * h->C * 27, 0.f, s));
     return 0;

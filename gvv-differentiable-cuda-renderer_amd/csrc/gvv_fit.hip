@@ -7,8 +7,8 @@
 //
 // Shape model: posed vertices v = R (T + sum_k alpha_k B_k - c) + c + t of a linear shape basis
 // under a rigid motion per subject, and the gradients of alpha and of (R, t).  The forward is
-// one thread per vertex for up to 16 subjects (the basis is read once per 16 subjects, the
-// coefficients and motions are wave-uniform scalar loads).  The backward sums over vertices:
+// one thread per vertex and group of 4 subjects (the coefficients and motions are wave-uniform
+// scalar loads).  The backward sums over vertices:
 // each workgroup takes a chunk of vertices, stages the chunk of the basis and the subjects'
 // unrotated vertex gradients in LDS and forms every (subject, coefficient) partial dot
 // product of the chunk; a second launch adds the chunks' partials in a fixed order.
@@ -99,32 +99,37 @@ __global__ void __launch_bounds__(kThreads) k_masked_sq_grad(const float* __rest
 }
 
 // ---- shape model -------------------------------------------------------------------------
-constexpr int kShapeSubjects = 16;     // subjects per pass (grid.y covers more)
+constexpr int kShapeSubjects = 16;     // subjects per backward workgroup (grid.y covers more)
+constexpr int kShapeMaxSubjects = 65535 * 4;   // the forward's grid.y limit at 4 subjects per thread
 constexpr int kShapeChunk = 64;        // vertices per backward workgroup
 constexpr int kShapeRow = 3 * kShapeChunk + 1;   // padded LDS row: lanes of one column differ in bank
 constexpr int kShapeMaxBasis = 128;
 constexpr int kShapeRt = 12;           // R row-major (9) and t (3) per subject
 
 // v[b][i] = R_b (T_i + sum_k alpha[b][k] basis[k][3i..3i+2] - c) + c + t_b and xc[b][i] = the
-// unrotated, centred vertex (T_i + ... - c), kept for the backward.  Thread = vertex; the block's
-// subjects are b0 .. b0 + nb - 1.
+// unrotated, centred vertex (T_i + ... - c), kept for the backward.  Thread = (vertex, group of
+// kShapeFwdSubjects subjects b0 .. b0 + nb - 1): enough wavefronts to cover the basis reads'
+// latency, which one thread per vertex for all subjects leaves exposed.
+constexpr int kShapeFwdSubjects = 4;
+
 __global__ void __launch_bounds__(64) k_shape_forward(const float* __restrict__ alpha, const float* __restrict__ rt,
                                                       const float* __restrict__ templ, const float* __restrict__ basis,
                                                       const float* __restrict__ centre, int B, int N, int K,
                                                       float* __restrict__ v, float* __restrict__ xc) {
   const int i = blockIdx.x * 64 + threadIdx.x;
-  const int b0 = blockIdx.y * kShapeSubjects;
-  const int nb = B - b0 < kShapeSubjects ? B - b0 : kShapeSubjects;
+  const int b0 = blockIdx.y * kShapeFwdSubjects;
+  const int nb = B - b0 < kShapeFwdSubjects ? B - b0 : kShapeFwdSubjects;
   if (i >= N) return;
-  float acc[kShapeSubjects][3];
+  float acc[kShapeFwdSubjects][3];
 #pragma unroll
-  for (int s = 0; s < kShapeSubjects; ++s) acc[s][0] = acc[s][1] = acc[s][2] = 0.f;
+  for (int s = 0; s < kShapeFwdSubjects; ++s) acc[s][0] = acc[s][1] = acc[s][2] = 0.f;
   const size_t n3 = (size_t)3 * N;
+  const float* bi = basis + 3 * (size_t)i;
+#pragma unroll 4
   for (int k = 0; k < K; ++k) {
-    const float* bk = basis + k * n3 + 3 * (size_t)i;
-    const float e0 = bk[0], e1 = bk[1], e2 = bk[2];
+    const float e0 = bi[k * n3], e1 = bi[k * n3 + 1], e2 = bi[k * n3 + 2];
 #pragma unroll
-    for (int s = 0; s < kShapeSubjects; ++s) {
+    for (int s = 0; s < kShapeFwdSubjects; ++s) {
       if (s < nb) {
         const float a = alpha[(size_t)(b0 + s) * K + k];
         acc[s][0] += a * e0;
@@ -136,7 +141,7 @@ __global__ void __launch_bounds__(64) k_shape_forward(const float* __restrict__ 
   const float c0 = centre[0], c1 = centre[1], c2 = centre[2];
   const float t0 = templ[3 * (size_t)i], t1 = templ[3 * (size_t)i + 1], t2 = templ[3 * (size_t)i + 2];
 #pragma unroll
-  for (int s = 0; s < kShapeSubjects; ++s) {
+  for (int s = 0; s < kShapeFwdSubjects; ++s) {
     if (s < nb) {
       const float* m = rt + (size_t)(b0 + s) * kShapeRt;
       const float x = (t0 + acc[s][0]) - c0, y = (t1 + acc[s][1]) - c1, z = (t2 + acc[s][2]) - c2;
@@ -169,10 +174,22 @@ __global__ void __launch_bounds__(256) k_shape_backward_partial(const float* __r
   const int nb = B - b0 < kShapeSubjects ? B - b0 : kShapeSubjects;
   const int stride = K + kShapeRt;
   const size_t n3 = (size_t)3 * N;
-  // the chunk of every basis row (coalesced along the row)
-  for (int q = tid; q < K * 3 * kShapeChunk; q += 256) {
-    const int k = q / (3 * kShapeChunk), c = q - k * 3 * kShapeChunk;
-    s_basis[k * kShapeRow + c] = c < 3 * nv ? basis[k * n3 + 3 * (size_t)i0 + c] : 0.f;
+  // the chunk of every basis row (coalesced along the row), four loads in flight per thread
+  const int total = K * 3 * kShapeChunk;
+  for (int q0 = tid; q0 < total; q0 += 4 * 256) {
+    float e[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int q = q0 + u * 256;
+      const int k = q / (3 * kShapeChunk), c = q - k * 3 * kShapeChunk;
+      e[u] = q < total && c < 3 * nv ? basis[k * n3 + 3 * (size_t)i0 + c] : 0.f;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int q = q0 + u * 256;
+      const int k = q / (3 * kShapeChunk), c = q - k * 3 * kShapeChunk;
+      if (q < total) s_basis[k * kShapeRow + c] = e[u];
+    }
   }
   // 16 lanes per subject, 4 vertices each: unrotated gradients into LDS, motion sums in registers
   {
@@ -312,14 +329,29 @@ __global__ void __launch_bounds__(256) k_pose_grad(const float* __restrict__ g_a
   g_out[o] = acc;
 }
 
-// out[b][q] = sum over chunks j (in order) of partial[j][b][q], q < K + 12: thread per (b, q).
-__global__ void __launch_bounds__(256) k_shape_backward_final(const float* __restrict__ partial, int chunks, int B,
-                                                              int stride, float* __restrict__ out) {
-  const int o = blockIdx.x * 256 + threadIdx.x;
-  if (o >= B * stride) return;
+// out[o] = sum over chunks j of partial[j][o], o = b * (K + 12) + q: a 1024-thread workgroup per
+// 64 outputs; thread (g, l) sums chunks g, g + 16, ... of output l, then thread (0, l) adds the
+// 16 group sums in order (deterministic).
+constexpr int kFinalGroups = 16;
+__global__ void __launch_bounds__(1024) k_shape_backward_final(const float* __restrict__ partial, int chunks, int B,
+                                                               int stride, float* __restrict__ out) {
+  __shared__ float s_sum[kFinalGroups][64];
+  const int l = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const int o = blockIdx.x * 64 + l;
+  const int outs = B * stride;
   float a = 0.f;
-  for (int j = 0; j < chunks; ++j) a += partial[(size_t)j * B * stride + o];
-  out[o] = a;
+  if (o < outs) {
+#pragma unroll 4
+    for (int j = g; j < chunks; j += kFinalGroups) a += partial[(size_t)j * outs + o];
+  }
+  s_sum[g][l] = a;
+  __syncthreads();
+  if (g == 0 && o < outs) {
+    float t = 0.f;
+#pragma unroll
+    for (int q = 0; q < kFinalGroups; ++q) t += s_sum[q][l];
+    out[o] = t;
+  }
 }
 
 bool check_launch(const char* what) {
@@ -367,7 +399,7 @@ extern "C" int gvv_masked_sq_error_grad(int32_t B, int64_t P, const float* rende
 }
 
 extern "C" int64_t gvv_shape_workspace_floats(int32_t B, int32_t N, int32_t K) {
-  if (B <= 0 || B > 65535 * kShapeSubjects || N <= 0 || K <= 0 || K > kShapeMaxBasis) return -1;
+  if (B <= 0 || B > kShapeMaxSubjects || N <= 0 || K <= 0 || K > kShapeMaxBasis) return -1;
   const int64_t chunks = (N + kShapeChunk - 1) / kShapeChunk;
   return (chunks + 1) * B * (K + kShapeRt);     // chunk partials, then their sums
 }
@@ -375,14 +407,14 @@ extern "C" int64_t gvv_shape_workspace_floats(int32_t B, int32_t N, int32_t K) {
 extern "C" int gvv_shape_forward(int32_t B, int32_t N, int32_t K, const float* alpha, const float* pose,
                                  float tscale, const float* templ, const float* basis, const float* centre, float* v,
                                  float* xc, float* rt, void* stream) {
-  if (B <= 0 || B > 65535 * kShapeSubjects || N <= 0 || K <= 0 || K > kShapeMaxBasis)
-    return gvv::set_last_error("shape forward: need B >= 1 subjects, N >= 1 vertices and 1 <= K <= 128 basis vectors");
+  if (B <= 0 || B > kShapeMaxSubjects || N <= 0 || K <= 0 || K > kShapeMaxBasis)
+    return gvv::set_last_error("shape forward: need 1 <= B <= 262140 subjects, N >= 1 vertices and 1 <= K <= 128 basis vectors");
   if (!alpha || !pose || !templ || !basis || !centre || !v || !xc || !rt)
     return gvv::set_last_error("shape forward: null pointer");
   hipStream_t s = (hipStream_t)stream;
   hipLaunchKernelGGL(k_pose_motion, dim3((unsigned)((B + 63) / 64)), dim3(64), 0, s, pose, tscale, B, rt);
   if (!check_launch("shape forward")) return 1;
-  const dim3 grid((unsigned)((N + 63) / 64), (unsigned)((B + kShapeSubjects - 1) / kShapeSubjects));
+  const dim3 grid((unsigned)((N + 63) / 64), (unsigned)((B + kShapeFwdSubjects - 1) / kShapeFwdSubjects));
   hipLaunchKernelGGL(k_shape_forward, grid, dim3(64), 0, s, alpha, rt, templ, basis, centre, B, N, K, v, xc);
   return check_launch("shape forward") ? 0 : 1;
 }
@@ -390,8 +422,8 @@ extern "C" int gvv_shape_forward(int32_t B, int32_t N, int32_t K, const float* a
 extern "C" int gvv_shape_backward(int32_t B, int32_t N, int32_t K, const float* g_v, const float* xc,
                                   const float* pose, const float* rt, float tscale, const float* basis, float* work,
                                   float* g_alpha_pose, void* stream) {
-  if (B <= 0 || B > 65535 * kShapeSubjects || N <= 0 || K <= 0 || K > kShapeMaxBasis)
-    return gvv::set_last_error("shape backward: need B >= 1 subjects, N >= 1 vertices and 1 <= K <= 128 basis vectors");
+  if (B <= 0 || B > kShapeMaxSubjects || N <= 0 || K <= 0 || K > kShapeMaxBasis)
+    return gvv::set_last_error("shape backward: need 1 <= B <= 262140 subjects, N >= 1 vertices and 1 <= K <= 128 basis vectors");
   if (!g_v || !xc || !pose || !rt || !basis || !work || !g_alpha_pose)
     return gvv::set_last_error("shape backward: null pointer");
   const int chunks = (N + kShapeChunk - 1) / kShapeChunk;
@@ -407,7 +439,7 @@ extern "C" int gvv_shape_backward(int32_t B, int32_t N, int32_t K, const float* 
   if (!check_launch("shape backward")) return 1;
   float* sums = work + (size_t)chunks * B * stride;
   const int64_t outs = (int64_t)B * stride;
-  hipLaunchKernelGGL(k_shape_backward_final, dim3((unsigned)((outs + 255) / 256)), dim3(256), 0, s, work, chunks, B,
+  hipLaunchKernelGGL(k_shape_backward_final, dim3((unsigned)((outs + 63) / 64)), dim3(1024), 0, s, work, chunks, B,
                      stride, sums);
   if (!check_launch("shape backward")) return 1;
   const int64_t outs6 = (int64_t)B * (K + 6);

@@ -60,14 +60,14 @@ extern "C" int gvv_backward_noalloc(void* handle, int32_t B, int32_t texH, int32
   if (check_device(h)) return 1;
   if (B <= 0) return set_last_error("batch size must be > 0");
   if (texH <= 0 || texW <= 0) return set_last_error("texture height/width must be > 0");
-  if (!g_pos || !g_col || !g_tex || !g_sh) return set_last_error("null gradient output pointer");
+  if (!g_pos || !g_tex || !g_sh) return set_last_error("null gradient output pointer");   // g_col may be NULL
   hipStream_t s = (hipStream_t)hip_stream;
   hipError_t e = hipSuccess;
   if (h->albedo == kNormal || h->albedo == kLighting) {
     // zero gradients, as gvv_backward (python/CudaRenderer.py:207-213); no scratch needed
     const int64_t nv = (int64_t)B * h->N * 3;
     if (e == hipSuccess) e = launch_fill_f32(g_pos, nv, 0.f, s);
-    if (e == hipSuccess) e = launch_fill_f32(g_col, nv, 0.f, s);
+    if (e == hipSuccess && g_col) e = launch_fill_f32(g_col, nv, 0.f, s);
     if (e == hipSuccess) e = launch_fill_f32(g_tex, (int64_t)B * texH * texW * 3, 0.f, s);
     if (e == hipSuccess) e = launch_fill_f32(g_sh, (int64_t)B * h->C * 27, 0.f, s);
     return e == hipSuccess ? 0 : set_last_error(std::string("gvv_backward_noalloc: ") + hipGetErrorString(e));

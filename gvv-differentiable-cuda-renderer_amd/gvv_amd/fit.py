@@ -121,7 +121,8 @@ class ShapeModel:
         dev = self.device
         self.template = torch.tensor(v * MM, dtype=torch.float32, device=dev)
         self.centre = self.template.mean(0)
-        self.basis = torch.tensor(Q.reshape(cfg.n_basis, -1), dtype=torch.float32, device=dev)   # (K, 3N)
+        # (K, 3N), row-major: torch.tensor keeps a numpy array's strides, and Q is a transposed view
+        self.basis = torch.tensor(np.ascontiguousarray(Q.reshape(cfg.n_basis, -1)), dtype=torch.float32, device=dev)
         col = r_col.uniform(0.2, 0.9, size=(self.N, 3))
         if cfg.colour_smoothing > 0:
             # low-frequency albedo: smoothed, then rescaled to the spread of U(0.2, 0.9) around 0.55

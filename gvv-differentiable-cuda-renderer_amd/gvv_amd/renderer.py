@@ -123,11 +123,13 @@ class Rasterizer:
         _lib.check(rc, "gvv_forward")
         return bary, face, render, vnormal, nmap
 
-    def backward(self, g_render, g_target, target, vpos, vcol, tex, sh, vnormal, face, extr, intr, bary=None):
+    def backward(self, g_render, g_target, target, vpos, vcol, tex, sh, vnormal, face, extr, intr, bary=None,
+                 vertex_color_grad=True):
         """Returns (vertex_pos_grad, vertex_color_grad, texture_grad, sh_coeff_grad).
 
         ``g_target`` may be None (zero: the model-to-data term is skipped); ``bary`` may be None
-        (barycentrics are recomputed from the face ids, bit-identical to the forward's)."""
+        (barycentrics are recomputed from the face ids, bit-identical to the forward's);
+        ``vertex_color_grad=False`` skips the vertex-colour gradient (returned as None)."""
         dev = vpos.device
         vpos, vcol, tex, sh, vnormal = (self._prep(x, dev) for x in (vpos, vcol, tex, sh, vnormal))
         B = vpos.shape[0]                                   # batch from vertex_pos dim 0 (CudaRendererGrad.cpp:195)
@@ -153,7 +155,7 @@ class Rasterizer:
             raise ValueError(f"vertex_normal has shape {tuple(vnormal.shape)}, expected {(B, C, N, 3)}")
         texH, texW = tex.shape[1], tex.shape[2]
         g_pos = torch.empty((B, N, 3), device=dev, dtype=torch.float32)
-        g_col = torch.empty((B, N, 3), device=dev, dtype=torch.float32)
+        g_col = torch.empty((B, N, 3), device=dev, dtype=torch.float32) if vertex_color_grad else None
         g_tex = torch.empty((B, texH, texW, 3), device=dev, dtype=torch.float32)
         g_sh = torch.empty((B, C, 27), device=dev, dtype=torch.float32)
         rc = _lib.load().gvv_backward(self._h, B, texH, texW, _ptr(g_render), _ptr(g_target), _ptr(target), _ptr(vpos),
@@ -228,7 +230,8 @@ class RenderFunction(torch.autograd.Function):
         if g_render is None and g_target is None:
             grads = (None, None, None, None)
         else:
-            grads = rast.backward(g_render, g_target, target, vpos, vcol, tex, sh, vnormal, face, extr, intr)
+            grads = rast.backward(g_render, g_target, target, vpos, vcol, tex, sh, vnormal, face, extr, intr,
+                                  vertex_color_grad=ctx.needs_input_grad[2])
         need = ctx.needs_input_grad
         out = [None]
         for i, g in enumerate(grads):

@@ -83,6 +83,9 @@ int gvv_forward(void* handle, int32_t B, int32_t texH, int32_t texW, const float
  *   bary      forward barycentrics, or NULL to recompute them from the face id (bit-identical
  *             to the forward's values)
  *   outputs   g_pos [B][N][3], g_col [B][N][3], g_tex [B][texH][texW][3], g_sh [B][C][27]
+ *             g_col may be NULL (an extension; the reference op always computes it): the
+ *             vertex-colour gradient is then neither formed nor written, which spares the
+ *             vertexColor backward a third of its per-face terms when colours are not fitted
  * In the normal and lighting modes every gradient is zero, as the reference's gradient
  * routing returns (python/CudaRenderer.py:207-213). */
 int gvv_backward(void* handle, int32_t B, int32_t texH, int32_t texW, const float* g_render, const float* g_target,
@@ -92,8 +95,8 @@ int gvv_backward(void* handle, int32_t B, int32_t texH, int32_t texW, const floa
 
 /* gvv_backward without any device allocation.  No counterpart in the reference: for callers
  * that cannot allocate device memory (code under HIP graph capture, tight serving loops,
- * harnesses that forbid hipMalloc).  Same arguments, results and zero-gradient modes as
- * gvv_backward.  The backward's scratch -- per-view cameras, the per-vertex position and
+ * harnesses that forbid hipMalloc).  Same arguments (g_col may be NULL), results and
+ * zero-gradient modes as gvv_backward.  The backward's scratch -- per-view cameras, the per-vertex position and
  * normal terms, the SH tile partials and the per-face constants, B*C*156 + B*N*24
  * + B*C*ceil(W/tile)*ceil(H/tile)*108 + B*F*128 bytes, each part rounded up to 256 -- lives in
  * one fixed 32 MiB device array per GPU that the library holds until the process ends; a call

@@ -131,3 +131,45 @@ def test_face_box_beyond_24_bits(device):
     s = scenes.single_triangle(width=8300, height=2100, z_m=0.8,
                                tri_m=((-2.0, -1.0, 0.0), (2.0, -1.0, 0.0), (0.0, 1.2, 0.0)))
     _run(s, device, "vertexColor", "shaded", m2d=False, check_grads=False)
+
+
+@pytest.mark.parametrize("m2d", [False, True])
+def test_backward_without_vertex_colour_gradient(device, m2d):
+    """g_col = NULL (Rasterizer.backward(vertex_color_grad=False)): the vertexColor backward
+    skips the colour terms; position, texture and SH gradients agree with the full call within
+    the gradient tolerance (the face slots then hold fewer terms, so float atomics add in
+    another order), and the no-allocation entry point accepts it too."""
+    s = scenes.config0(width=128, height=128, level=3)
+    kw = s.renderer_kwargs()
+    rast = Rasterizer(**kw)
+    ins = [_t(a, device) for a in s.inputs()]
+    vpos, vcol, tex, sh, extr, intr = ins
+    bary, face, render, vnormal, _ = rast.forward(*ins)
+    g = torch.randn(render.shape, generator=torch.Generator(device="cpu").manual_seed(5)).to(device)
+    gt = tgt = None
+    if m2d:
+        gt = torch.randn(render.shape, generator=torch.Generator(device="cpu").manual_seed(6)).to(device)
+        tgt = torch.rand(render.shape, generator=torch.Generator(device="cpu").manual_seed(7)).to(device)
+    full = rast.backward(g, gt, tgt, vpos, vcol, tex, sh, vnormal, face, extr, intr)
+    part = rast.backward(g, gt, tgt, vpos, vcol, tex, sh, vnormal, face, extr, intr, vertex_color_grad=False)
+    assert part[1] is None
+    for name, a, b in (("g_pos", full[0], part[0]), ("g_tex", full[2], part[2]), ("g_sh", full[3], part[3])):
+        e = rel_err(b.cpu().numpy(), a.cpu().numpy())
+        print(f"no colour gradient m2d={m2d}: {name} rel err {e:.2e}", flush=True)
+        assert e <= GRAD_TOL
+    assert float(full[1].abs().max()) > 0.0
+    # the no-allocation entry point with g_col = NULL
+    import ctypes
+    from gvv_amd import _lib
+    L = _lib.load()
+    B, N = vpos.shape[0], vpos.shape[1]
+    gp = torch.empty((B, N, 3), device=device)
+    gx = torch.empty_like(part[2])
+    gs = torch.empty_like(part[3])
+    p = lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else None
+    rc = L.gvv_backward_noalloc(rast._h, B, tex.shape[1], tex.shape[2], p(g), p(gt), p(tgt), p(vpos), p(vcol), p(tex),
+                                p(sh), p(vnormal), None, p(face), p(extr.reshape(B, -1)), p(intr.reshape(B, -1)),
+                                p(gp), None, p(gx), p(gs), ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+    _lib.check(rc, "gvv_backward_noalloc")
+    torch.cuda.synchronize()
+    assert rel_err(gp.cpu().numpy(), full[0].cpu().numpy()) <= GRAD_TOL
