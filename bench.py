@@ -339,7 +339,7 @@ def main():
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print(f"bench.py measures one GPU: every (batch, view) pair renders independently and this build "
+        print(f"bench.py measures one GPU: every (batch, view) pair renders independently and the renderer "
               f"runs on a single MI355X; --gpus {args.gpus} (WORLD_SIZE={world}) is not supported.",
               file=sys.stderr)
         sys.exit(2)
