@@ -17,9 +17,10 @@ the proposal-warm-started fit against the plain fit:
 
   fit      35,002-vertex / 70,000-triangle template, 80-dim shape basis, 27 SH, 6-DoF pose;
            16 subjects x 4 views at 512^2, masked L2, Adam lr 1e-2, 200 iterations (plain,
-           zero-initialised); the proposal network is trained first on renders generated on
-           the fly (--train-samples, default 200,000), then the warm-started fit runs until it
-           reaches the plain fit's final loss.  Reported: both wall-clock times and their ratio.
+           zero-initialised); the proposal network is trained first on renders the renderer
+           generates once and keeps in device memory (--train-samples, default 200,000; four
+           passes over them), then the warm-started fit runs until it reaches the plain fit's
+           final loss.  Reported: both wall-clock times and their ratio.
 
 --workload multi instead times the batched uneven workload (32 meshes, 2k-200k triangles,
 1-8 views at 256/512/1024^2) through the multi-mesh runtime: autotuned, one hipGraph launch
