@@ -24,7 +24,7 @@ the proposal-warm-started fit against the plain fit:
 
 --workload multi instead times the batched uneven workload (32 meshes, 2k-200k triangles,
 1-8 views at 256/512/1024^2) through the multi-mesh runtime: autotuned, one hipGraph launch
-per forward+backward step.
+per forward+backward step.  --workload fit times iterations of the plain fit alone.
 """
 from __future__ import annotations
 
@@ -319,6 +319,49 @@ def bench_multi(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_fit(args):
+    """--workload fit: the plain analysis-by-synthesis fit of the fit configuration (16 subjects x
+    4 views at 512^2, 70,000-triangle template, 80 shape coefficients, masked L2, Adam), timed
+    over `steps` iterations after `warmup`; one step = one iteration (shape model, render,
+    loss, backward, optimizer step, the loss read on the host)."""
+    import numpy as np
+    import torch
+    from gvv_amd import fit
+
+    dev = torch.device("cuda", 0)
+    cfg = fit.FitConfig()
+    model = fit.ShapeModel(cfg, dev)
+    gt = fit.sample_params(np.random.Generator(np.random.PCG64(12345)), cfg.batch, cfg.n_basis, dev)
+    target, mask = fit.make_targets(model, gt)
+    init = fit.plain_init(cfg.batch, cfg.n_basis, dev)
+    fit.fit(model, target, mask, init, iters=max(1, args.warmup))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = fit.fit(model, target, mask, init, iters=args.steps)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    px = cfg.batch * cfg.views * cfg.resolution * cfg.resolution
+    line = {
+        "metric": "fit iterations/s (16 subjects x 4 views at 512^2, 70k triangles)",
+        "value": round(1e3 / ms, 2),
+        "unit": "iterations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded shape model and ground-truth parameters)",
+        "config": {"workload": "fit16x4x512_vertexcolor_shaded", "subjects": cfg.batch, "views": cfg.views,
+                   "resolution": cfg.resolution, "template_vertices": model.N, "template_triangles": model.F,
+                   "basis": cfg.n_basis, "render_mpix_per_s": round(px / (ms / 1e3) / 1e6, 1),
+                   "loss_first": res.losses[0], "loss_last": res.final_loss},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -330,7 +373,7 @@ def main():
     ap.add_argument("--tuning", default=None, metavar="TILE,BOX",
                     help="fix the backward tile edge and large-face threshold instead of choosing them")
     ap.add_argument("--kernels-json", default=None, help="also write per-kernel timings here")
-    ap.add_argument("--workload", choices=("render", "multi"), default="render")
+    ap.add_argument("--workload", choices=("render", "multi", "fit"), default="render")
     ap.add_argument("--no-fit", action="store_true", help="skip the proposal + fit time-to-tolerance run")
     ap.add_argument("--no-multi", action="store_true", help="skip the batched uneven-mesh figures")
     ap.add_argument("--train-samples", type=int, default=200_000)
@@ -347,6 +390,8 @@ def main():
 
     if args.workload == "multi":
         return bench_multi(args)
+    if args.workload == "fit":
+        return bench_fit(args)
 
     import numpy as np
     import torch

@@ -167,15 +167,26 @@ class ShapeModel:
         B = p.alpha.shape[0]
         C, R = self.cfg.views, self.cfg.resolution
         vpos = self.vertices(p)
-        vcol = self.colors.expand(B, self.N, 3)
         sh = p.sh[:, None, :].expand(B, C, 27)
-        tex = self._tex.expand(B, 1, 1, 3)
         key = B
         if key not in self._target_dummy:
             self._target_dummy[key] = torch.zeros((B, C, R, R, 3), device=self.device)
-        return RenderFunction.apply(self.rast, vpos, vcol, tex, sh, self._target_dummy[key],
-                                    self.extr.expand(B, C, 12).reshape(B, C * 12),
-                                    self.intr.expand(B, C, 9).reshape(B, C * 9))
+        return RenderFunction.apply(self.rast, vpos, self._batched(self.colors, B, self.N, 3),
+                                    self._batched(self._tex, B, 1, 1, 3), sh, self._target_dummy[key],
+                                    self._batched(self.extr, B, C, 12).view(B, C * 12),
+                                    self._batched(self.intr, B, C, 9).view(B, C * 9))
+
+    def _batched(self, t, B, *shape):
+        """The per-subject copy [B, *shape] of a tensor shared by all subjects (colours, texture,
+        cameras), made contiguous once and kept while `t` is the same, unmodified tensor: the
+        renderer takes contiguous batches, and expanding them every iteration costs two copies
+        per tensor (forward and backward)."""
+        cache = self.__dict__.setdefault("_batched_cache", {})
+        hit = cache.get((id(t), B))
+        if hit is None or hit[0] is not t or hit[1] != t._version:
+            hit = (t, t._version, t.expand(B, *shape).contiguous())
+            cache[(id(t), B)] = hit
+        return hit[2]
 
 
 def sample_params(rng: np.random.Generator, B: int, n_basis: int, device) -> Params:
