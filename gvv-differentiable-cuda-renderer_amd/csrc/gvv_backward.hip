@@ -299,12 +299,14 @@ __device__ __forceinline__ void pixel_terms(const BackwardArgs& a, const Camera&
 
 // One wavefront per TILE x TILE pixel tile of one (b, c) image (TILE = 8, 16 or 32), in
 // (TILE/8)^2 rounds of 8x8 pixels.
-// Per-face terms: every distinct face of the tile gets a slot in LDS (found by a ballot over
-// the slot table); lanes add their terms into their face's slot with LDS atomics, and the
+// Per-face terms: every distinct face of the tile gets a slot in LDS (every lane finds or
+// inserts its face in an LDS hash table at once, and the inserting lanes number the new slots
+// in lane order); lanes add their terms into their face's slot with LDS atomics, and the
 // slots are flushed with lane-parallel global atomics (one per face, vertex and component).
 // SH terms accumulate in registers over the tile and are reduced once into the tile's
 // partial slots.
 constexpr int kFaceSlots = 96;
+constexpr int kHashCap = 128;
 
 // COL_GRAD: the vertex-colour gradient is formed (vertexColor mode with a g_col output).
 template <int ALBEDO, bool SHADED, bool M2D, bool HAVE_BARY, int TILE, bool COL_GRAD>
@@ -318,6 +320,8 @@ __device__ __forceinline__ void pixel_grad_body(const BackwardArgs& a, int64_t w
   constexpr int NT1 = NT > 0 ? NT : 1;
   __shared__ float s_acc[kFaceSlots][NT1];
   __shared__ int s_face[kFaceSlots];
+  // face id -> slot: open addressing with linear probing, keys face + 1 (0: free)
+  __shared__ int2 s_hash[kHashCap];
   constexpr int TS = ALBEDO == kTextured ? 64 : 1;
   __shared__ int s_tkey[TS];          // a round's texture-gradient terms, compacted
   __shared__ float s_tval[TS * 3];
@@ -354,10 +358,10 @@ __device__ __forceinline__ void pixel_grad_body(const BackwardArgs& a, int64_t w
   for (int k = 0; k < 27; ++k) sh_acc[k] = 0.f;
   bool any = false;
   int nslots = 0;
-  // the face of every slot, also held in registers for the lookups: lane j holds slot j in
-  // slot_face0 and slot 64 + j in slot_face1 (-1: free)
-  int slot_face0 = -1, slot_face1 = -1;
-  static_assert(kFaceSlots <= 128, "two registers of slot faces per lane");
+  static_assert(kHashCap % 64 == 0 && kHashCap > kFaceSlots, "hash table: whole lanes, spare capacity");
+#pragma unroll
+  for (int q = 0; q < kHashCap / 64; ++q) s_hash[lane + 64 * q].x = 0;
+  __syncthreads();
 
   auto flush = [&]() {
     __syncthreads();
@@ -373,9 +377,10 @@ __device__ __forceinline__ void pixel_grad_body(const BackwardArgs& a, int64_t w
       else dst = (grp * 9 == G_POS && NEED_POS ? a.g_pos : a.g_col) + row * 3 + kk % 3;
       atomicAdd(dst, v);
     }
+#pragma unroll
+    for (int q = 0; q < kHashCap / 64; ++q) s_hash[lane + 64 * q].x = 0;
     __syncthreads();
     nslots = 0;
-    slot_face0 = slot_face1 = -1;
   };
 
   // Software pipeline over the rounds: the next round's face id, and then its vertex ids, are
@@ -451,34 +456,31 @@ __device__ __forceinline__ void pixel_grad_body(const BackwardArgs& a, int64_t w
       const int runs = __popcll(__ballot(idf >= 0 && idf != prev));
       if (nslots + runs > kFaceSlots) flush();
     }
-    int myslot = -1;
-    while (pending) {
-      const int leader = __ffsll((unsigned long long)pending) - 1;
-      const int f = __builtin_amdgcn_readlane(idf, leader);
-      const bool mine = idf == f;
-      pending &= ~__ballot(mine);
-      // lookup in the register copy of the slot table: compares and ballots, no LDS round trip
-      int slot;
-      const uint64_t hit0 = __ballot(slot_face0 == f);
-      if (hit0) {
-        slot = __ffsll((unsigned long long)hit0) - 1;
-      } else {
-        const uint64_t hit1 = __ballot(slot_face1 == f);
-        if (hit1) {
-          slot = 64 + __ffsll((unsigned long long)hit1) - 1;
-        } else {
-          slot = nslots++;
-          if (slot < 64) {
-            if (lane == slot) slot_face0 = f;
-          } else if (lane == slot - 64) {
-            slot_face1 = f;
-          }
-          if (lane == 0) s_face[slot] = f;
-          for (int k = lane; k < NT; k += 64) s_acc[slot][k] = 0.f;
-        }
+    // every lane finds or inserts its face in the hash table at once; the lanes that inserted a
+    // face number their new slots in lane order
+    int myslot = -1, h = 0;
+    bool inserted = false;
+    if (idf >= 0) {
+      const int key = idf + 1;
+      h = (int)(((unsigned)idf * 2654435761u) >> 25) & (kHashCap - 1);
+      for (;;) {
+        const int old = atomicCAS(&s_hash[h].x, 0, key);
+        if (old == 0) { inserted = true; break; }
+        if (old == key) break;
+        h = (h + 1) & (kHashCap - 1);
       }
-      if (mine) myslot = slot;
     }
+    const uint64_t ins = __ballot(inserted);
+    const int nnew = __popcll(ins);
+    if (inserted) {
+      const int slot = nslots + __builtin_amdgcn_mbcnt_hi((unsigned)(ins >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)ins, 0u));
+      s_hash[h].y = slot;
+      s_face[slot] = idf;
+    }
+    for (int q = lane; q < nnew * NT; q += 64) s_acc[nslots + q / NT][q % NT] = 0.f;
+    nslots += nnew;
+    __syncthreads();
+    if (idf >= 0) myslot = s_hash[h].y;
     // lanes are 8x8 pixels (x = lane & 7, y = lane >> 3): merge x neighbours (quad_perm
     // [1,0,3,2]), y neighbours (row_ror:8), x pairs two apart (quad_perm [2,3,0,1]), then
     // four apart (row_shl:4 / row_shr:4); survivors hold the sums of up to 16 lanes
