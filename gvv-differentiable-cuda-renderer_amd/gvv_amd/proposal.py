@@ -379,7 +379,7 @@ def warm_fit(net: ProposalNet, model: fit.ShapeModel, target, mask, iters: int, 
 
 
 def compare_fits(net: ProposalNet, model: fit.ShapeModel, gt: fit.Params, iters: int | None = None,
-                 plain_runs: int = 3, warm_runs: int = 3) -> WarmStartReport:
+                 plain_runs: int = 5, warm_runs: int = 5) -> WarmStartReport:
     """Plain fit for `iters` Adam steps (`plain_runs` times: float atomics make its last bits, and
     so its final loss, vary from run to run; the run with the median final loss is the
     reference), then the warm-started fit (`warm_runs` times) until it reaches that run's final
