@@ -136,6 +136,7 @@ class ShapeModel:
                                albedo_mode="vertexColor", shading_mode="shaded")
         self._tex = torch.full((1, 1, 1, 3), 0.5, device=dev)
         self._target_dummy = {}
+        self._batched_cache = {}
 
     # -- geometry ------------------------------------------------------------------------------
     def motion(self, p: Params) -> torch.Tensor:
@@ -181,7 +182,7 @@ class ShapeModel:
         cameras), made contiguous once and kept while `t` is the same, unmodified tensor: the
         renderer takes contiguous batches, and expanding them every iteration costs two copies
         per tensor (forward and backward)."""
-        cache = self.__dict__.setdefault("_batched_cache", {})
+        cache = self._batched_cache
         hit = cache.get((id(t), B))
         if hit is None or hit[0] is not t or hit[1] != t._version:
             hit = (t, t._version, t.expand(B, *shape).contiguous())
