@@ -399,6 +399,8 @@ __device__ __forceinline__ void raster_body(const RasterArgs& ra, int64_t wg) {
   // tiny boxes: each lane walks its own box; pixel p of every lane's face per iteration
   const int my_area = small && bw * bh <= kTinyBox ? bw * bh : 0;
   int dx = 0, dy = 0;     // pixel p of this lane's box, stepped along its rows
+  EdgeWalk walk;
+  walk.start(et, fx0, fy0, bw);
   for (int p = 0; __ballot(p < my_area); ++p) {
     if (count > kCandCap - 64) {
       __syncthreads();
@@ -406,9 +408,11 @@ __device__ __forceinline__ void raster_body(const RasterArgs& ra, int64_t wg) {
       __syncthreads();
       count = 0;
     }
-    const bool pass = p < my_area && et.pass(fx0 + dx + 0.5f, fy0 + dy + 0.5f);
+    const bool pass = p < my_area && walk.pass(et.on);
     const unsigned word = cand_word(cand_packed(W, H), false, lane, fx0 + dx, fy0 + dy, dx, dy, bw);
-    if (++dx == bw) { dx = 0; ++dy; }
+    const bool row_end = ++dx == bw;
+    if (row_end) { dx = 0; ++dy; }
+    walk.step(et, row_end);
     const uint64_t m = __ballot(pass);
     if (pass) {
       const int off = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
@@ -430,7 +434,7 @@ __device__ __forceinline__ void raster_body(const RasterArgs& ra, int64_t wg) {
   flush_candidates(st, own, cand, count, cams, one_view, bc0, keys, W, H, false);
 }
 
-__global__ void __launch_bounds__(64) k_raster(RasterArgs ra) { raster_body(ra, xcd_contiguous(blockIdx.x, gridDim.x)); }
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) k_raster(RasterArgs ra) { raster_body(ra, xcd_contiguous(blockIdx.x, gridDim.x)); }
 
 // The depth pass's large-face test alone, so that the large-face pass can run beside the depth
 // pass: the (view, face) items whose bounding boxes exceed the threshold are appended to the
@@ -461,7 +465,7 @@ __global__ void __launch_bounds__(64) k_classify_large(RasterArgs ra) {
   }
 }
 
-__global__ void __launch_bounds__(64) k_raster_multi(MeshTable<RasterArgs> t) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) k_raster_multi(MeshTable<RasterArgs> t) {
   const MeshSlice m = t.find((unsigned)xcd_contiguous(blockIdx.x, gridDim.x));
   const RasterArgs a = t.load(m.mesh);
   raster_body(a, m.local);

@@ -49,6 +49,28 @@ GVV_HD EdgeTest edge_test(float4 p0, float4 p1, float4 p2) {
   return e;
 }
 
+// The pre-test walked along the rows of a box: the three edge values at the current pixel are
+// kept and advanced by a (one pixel along a row) or, at a row end, by b - (bw - 1) a (back to
+// the row's first pixel, one row down), instead of being evaluated anew at every pixel.  The
+// stepped values differ from the direct ones by rounding only (at most 64 steps of values
+// under 100 px), far inside the margin; tests/edge_margin_check.hip runs this walk over every
+// box it draws and counts the accepted pixels that it would reject (none).
+struct EdgeWalk {
+  float e[3];     // edge values at the current pixel
+  float w[3];     // row-end step b - (bw - 1) a
+  GVV_HD void start(const EdgeTest& t, int x0, int y0, int bw) {
+    const float x = x0 + 0.5f, y = y0 + 0.5f;
+    for (int i = 0; i < 3; ++i) {
+      e[i] = t.a[i] * x + t.b[i] * y + t.c[i];
+      w[i] = t.b[i] - (float)(bw - 1) * t.a[i];
+    }
+  }
+  GVV_HD bool pass(bool on) const { return !on || (e[0] >= 0.f && e[1] >= 0.f && e[2] >= 0.f); }
+  GVV_HD void step(const EdgeTest& t, bool row_end) {
+    for (int i = 0; i < 3; ++i) e[i] += row_end ? w[i] : t.a[i];
+  }
+};
+
 // Conservative x-range [xl, xr] of bounding-box row y (pixel centres at y + 0.5) inside the
 // three half-planes of the edge pre-test, widened by one pixel on each side so that float
 // rounding of the bounds never drops a pixel the pre-test would pass.  Without a usable edge

@@ -2,7 +2,10 @@
 // pixel lie that the exact ray/triangle test accepts?  Random triangles of 0.3-100 mm, a third
 // of them slivers, seen by pinhole cameras (f = 1.2 W, 1024^2) from 0.6-5 m at every
 // orientation; prints the most negative raw edge distance of an accepted pixel, which the
-// pre-test's margin (gvv_raster.hpp kEdgeMargin) must exceed.  Host only.
+// pre-test's margin (gvv_raster.hpp kEdgeMargin) must exceed.  Boxes of up to 64 pixels are
+// also walked as the depth pass walks them (EdgeWalk, edge values stepped along the rows), and
+// every pixel the exact test accepts but the walk would reject is counted (walk_missed).
+// Host only.
 #include <cstdio>
 #include <cstdlib>
 #include <random>
@@ -27,6 +30,8 @@ int main(int argc, char** argv) {
   const int W = 1024, H = 1024;
   double worst = 1e9; long long accepted = 0, tested = 0;
   double worst_size = 0;
+  long long walk_boxes = 0, walk_missed = 0;
+  double walk_dev = 0;
   for (int t = 0; t < ntri; ++t) {
     double pos[3] = {G(rng), G(rng) * 0.3, G(rng)}; double n = std::sqrt(pos[0]*pos[0]+pos[1]*pos[1]+pos[2]*pos[2]);
     const double dist = (t % 4 == 0 ? 600.0 : 2000.0) + 3000.0 * U(rng);
@@ -51,20 +56,34 @@ int main(int argc, char** argv) {
     const EdgeTest et = edge_test(p[0], p[1], p[2]);
     if (!et.on) continue;
     const TriSetup ts = tri_setup(v[0], v[1], v[2]);
+    const int bw = bx.x1 - bx.x0 + 1;
+    const bool walked = (long long)bw * (bx.y1 - bx.y0 + 1) <= 64;
+    EdgeWalk walk;
+    walk.start(et, bx.x0, bx.y0, bw);
+    if (walked) ++walk_boxes;
     for (int y = bx.y0; y <= bx.y1; ++y)
       for (int x = bx.x0; x <= bx.x1; ++x) {
         ++tested;
+        const bool walk_pass = walk.pass(et.on);
+        if (walked) {
+          for (int i = 0; i < 3; ++i) {
+            const double dev = std::fabs((double)walk.e[i] - ((double)et.a[i] * (x + 0.5) + (double)et.b[i] * (y + 0.5) + (double)et.c[i]));
+            if (dev > walk_dev) walk_dev = dev;
+          }
+        }
+        walk.step(et, x == bx.x1);
         const V3 d = pixel_ray(cam, x + 0.5f, y + 0.5f);
         const V3 abc = ray_barycentric(ts, cam_origin_scaled(cam), d);
         if (!inside(abc)) continue;
         ++accepted;
+        if (walked && !walk_pass) ++walk_missed;
         for (int i = 0; i < 3; ++i) {
           const double e = (double)et.a[i] * (x + 0.5) + (double)et.b[i] * (y + 0.5) + (double)et.c[i] - kEdgeMargin;
           if (e < worst) { worst = e; worst_size = size; }
         }
       }
   }
-  printf("tested %lld accepted %lld worst %.6e margin %.6e size %.2f\n", tested, accepted, worst, (double)kEdgeMargin,
-         worst_size);
+  printf("tested %lld accepted %lld worst %.6e margin %.6e size %.2f walk_boxes %lld walk_missed %lld walk_dev %.3e\n",
+         tested, accepted, worst, (double)kEdgeMargin, worst_size, walk_boxes, walk_missed, walk_dev);
   return 0;
 }

@@ -1,7 +1,8 @@
 """The depth pass's edge pre-test keeps every pixel the exact ray/triangle test accepts: over
 about 200 M pixel tests of random triangles at the renderer's scale (slivers, near and far
 cameras, every orientation), the most negative raw edge distance of an accepted pixel stays
-far inside the pre-test's margin (checked on the host with the kernels' own header)."""
+far inside the pre-test's margin (checked on the host with the kernels' own header), and the
+stepped walk of small boxes keeps every accepted pixel."""
 import os
 import subprocess
 
@@ -26,3 +27,8 @@ def test_pretest_margin_covers_exact_test(tmp_path):
     assert int(vals["accepted"]) > 10_000_000
     worst, margin = float(vals["worst"]), float(vals["margin"])
     assert worst > -margin / 20.0, (worst, margin)
+    # the stepped walk of boxes up to 64 px rejects no accepted pixel, and strays from the
+    # direct edge values by far less than the margin
+    assert int(vals["walk_boxes"]) > 100_000
+    assert int(vals["walk_missed"]) == 0
+    assert float(vals["walk_dev"]) < margin / 20.0
